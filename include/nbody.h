@@ -1,0 +1,254 @@
+/*
+ * nbody.h - C ABI of the MI355X (gfx950) gravitational N-body step.
+ *
+ * This library is a drop-in for the simulator behind the reference's
+ * `INBodySim` interface (src/Sim/INBodySim.hpp:11-28):
+ *
+ *     class INBodySim {
+ *         virtual void Init(std::vector<Particle>& particles) = 0;
+ *         virtual void Update(float dt) = 0;
+ *         virtual void RenderDebug(Matrix view, Matrix proj) {}
+ *     };
+ *     std::unique_ptr<INBodySim> CreateNBodySim(ID3D11DeviceContext*, ENBodySim);
+ *     std::string NBodySimGetName(ENBodySim);
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, returns
+ * an `nbody_status` and never throws.  A failed call leaves a message that
+ * `nbody_last_error()` returns (per calling thread).  One simulation handle
+ * must not be used from two threads at once; distinct handles are
+ * independent.  All device work of a handle runs on the handle's own HIP
+ * stream; `nbody_step` only enqueues, every read synchronises.
+ *
+ * The header-only C++ adapter `nbody_sim.hpp` re-exposes the reference's
+ * Init/Update/read-in-place contract on top of these calls.
+ */
+#ifndef NBODY_H_
+#define NBODY_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NBODY_ABI_VERSION 1
+
+/* ------------------------------------------------------------------------ */
+/* Particle record                                                           */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Byte-compatible with the reference's `struct Particle`
+ * (src/Render/Misc/Particle.hpp:8-18) as laid out by a 64-bit compiler:
+ * Vector3 Position @0, Color Colour @12, Color OriginalColour @28,
+ * 4 bytes of padding @44, Vec3d Velocity @48, Vec3d Forces @72,
+ * double Mass @96; 104 bytes in all.  It is also the record of the
+ * reference's `.nbody` snapshot file (SimulationState.cpp:317-331).
+ */
+typedef struct nbody_particle {
+    float  position[3];
+    float  colour[4];
+    float  original_colour[4];
+    float  pad_;
+    double velocity[3];
+    double forces[3];
+    double mass;
+} nbody_particle;
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------ */
+
+typedef enum nbody_status {
+    NBODY_OK = 0,
+    NBODY_ERR_INVALID_ARG = 1,     /* null pointer, bad size, bad enum value     */
+    NBODY_ERR_OUT_OF_MEMORY = 2,   /* device or host allocation failed            */
+    NBODY_ERR_DEVICE = 3,          /* a HIP call failed                           */
+    NBODY_ERR_NOT_INITIALISED = 4, /* step/read before init                       */
+    NBODY_ERR_UNSUPPORTED = 5,     /* valid request this build does not serve     */
+    NBODY_ERR_CAPACITY = 6,        /* request exceeds a fixed capacity            */
+    NBODY_ERR_IO = 7               /* snapshot file could not be read or written  */
+} nbody_status;
+
+/* ------------------------------------------------------------------------ */
+/* Simulation parameters                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces `enum class ENBodySim` (src/Sim/INBodySim.hpp:11-17): both of the
+ * reference's direct-sum simulators (BruteForceCPU, BruteForceGPU) map to
+ * NBODY_METHOD_DIRECT; BarnesHut maps to NBODY_METHOD_BARNES_HUT. */
+typedef enum nbody_method {
+    NBODY_METHOD_DIRECT = 1,
+    NBODY_METHOD_BARNES_HUT = 2
+} nbody_method;
+
+/* Pairwise force law.
+ * NBODY_LAW_R is the reference's law (src/Sim/Physics.hpp:25-35 with
+ * src/Sim/BruteForceCPU.cpp:36-40):  a_i = sum_j G m_j rhat_ij / (r^2 + S),
+ * where `softening` is S and rhat is the unit vector from i to j.
+ * NBODY_LAW_P is Plummer: a_i = sum_j G m_j r_ij / (r^2 + eps^2)^(3/2), where
+ * `softening` is eps^2 (must be > 0). */
+typedef enum nbody_law {
+    NBODY_LAW_R = 0,
+    NBODY_LAW_P = 1
+} nbody_law;
+
+/* Barnes-Hut root cell. FIXED uses root_lo/root_size as the reference's
+ * fixed cube (src/Sim/BarnesHut.cpp:14-19); DYNAMIC takes the bounding box of
+ * the particles at every step and rounds it out to a power-of-two cube. */
+typedef enum nbody_root_mode {
+    NBODY_ROOT_DYNAMIC = 0,
+    NBODY_ROOT_FIXED = 1
+} nbody_root_mode;
+
+typedef struct nbody_config {
+    int32_t method;      /* nbody_method                                        */
+    int32_t law;         /* nbody_law                                           */
+    double  G;           /* gravitational constant                              */
+    double  softening;   /* S (law R) or eps^2 (law P)                          */
+    double  scale;       /* drift divisor: x += v*dt/scale (Physics.hpp:16)     */
+    double  theta;       /* Barnes-Hut opening angle (Octree.cpp:5)             */
+    int32_t root_mode;   /* nbody_root_mode                                     */
+    float   root_lo[3];  /* fixed root: lower corner                            */
+    float   root_size;   /* fixed root: edge length                             */
+    int32_t device;      /* HIP device ordinal                                  */
+    int32_t tune_i_per_lane; /* direct kernel: targets per lane (0 = auto)      */
+    int32_t tune_block;      /* direct kernel: threads per workgroup (0 = auto) */
+    int32_t tune_split;      /* direct kernel: j-chunks per target (0 = auto)   */
+    int32_t reserved[8];
+} nbody_config;
+
+/* G = 1, law P, eps^2 = 1e-4, scale = 1, theta = 0.5, dynamic root,
+ * device 0, autotuned launch shape. */
+void nbody_config_default(nbody_config* cfg);
+
+/* The reference's constants (src/Sim/Physics.hpp:9-16, Octree.cpp:5,
+ * BarnesHut.cpp:14-19): law R, G = 6.674e-11, S = 10, scale = 2.3e13,
+ * theta = 2.0, fixed root [-4000, 4000)^3. */
+void nbody_config_reference(nbody_config* cfg, int32_t method);
+
+/* ------------------------------------------------------------------------ */
+/* Simulation handle (INBodySim)                                             */
+/* ------------------------------------------------------------------------ */
+
+typedef struct nbody_sim nbody_sim;
+
+/* CreateNBodySim (src/Sim/INBodySim.cpp:7-27). */
+nbody_status nbody_create(const nbody_config* cfg, nbody_sim** out);
+nbody_status nbody_destroy(nbody_sim* sim);
+
+/* INBodySim::Init (src/Sim/INBodySim.hpp:21): copies n particles to the
+ * device (no host pointer is kept).  May be called again with a new n. */
+nbody_status nbody_init(nbody_sim* sim, const nbody_particle* particles, size_t n);
+
+/* Same, from structure-of-arrays host buffers: pos[3n], vel[3n] (may be
+ * NULL for zero velocity), mass[n]. */
+nbody_status nbody_init_soa(nbody_sim* sim, const float* pos, const float* vel,
+                            const double* mass, size_t n);
+
+/* INBodySim::Update (src/Sim/INBodySim.hpp:22) applied nsteps times:
+ * kick then drift, v += a*dt; x += v*dt/scale (BruteForceCPU.cpp:61-73).
+ * Enqueued on the handle's stream; returns before the device finishes. */
+nbody_status nbody_step(nbody_sim* sim, float dt, int32_t nsteps);
+
+/* Wait for every step enqueued on the handle. */
+nbody_status nbody_synchronize(nbody_sim* sim);
+
+#define NBODY_FIELD_POSITION 1u
+#define NBODY_FIELD_VELOCITY 2u
+#define NBODY_FIELD_FORCES   4u
+
+/* Write the selected fields of the current state into the caller's records
+ * (the reference's in-place update of `std::vector<Particle>`,
+ * SimulationState.cpp:52-60).  FORCES is m * a of the last force evaluation
+ * in FP64.  Other fields of the records are left untouched. */
+nbody_status nbody_read(nbody_sim* sim, nbody_particle* particles, size_t n, uint32_t fields);
+
+/* Structure-of-arrays read: any pointer may be NULL.  acc is the last
+ * acceleration evaluated (3n floats). */
+nbody_status nbody_read_soa(nbody_sim* sim, float* pos, float* vel, float* acc, size_t n);
+
+/* Evaluate accelerations at the current positions without stepping; acc
+ * receives 3n floats.  Used for parity checks. */
+nbody_status nbody_compute_accelerations(nbody_sim* sim, float* acc, size_t n);
+
+/* Number of particles of the last init. */
+nbody_status nbody_size(nbody_sim* sim, size_t* n);
+
+/* ------------------------------------------------------------------------ */
+/* Diagnostics and timing                                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct nbody_diag {
+    double kinetic;              /* sum 1/2 m v^2                                    */
+    double potential;            /* sum over pairs of the law's pair potential        */
+    double total;                /* kinetic + potential                              */
+    double momentum[3];          /* sum m v                                          */
+    double angular_momentum[3];  /* sum m x cross v                                   */
+    double mass;                 /* sum m                                            */
+} nbody_diag;
+
+/* FP64 energy and momenta of the current state, computed on the device. */
+nbody_status nbody_diagnostics(nbody_sim* sim, nbody_diag* out);
+
+typedef struct nbody_timing {
+    int32_t steps;              /* steps in the last nbody_step call               */
+    int32_t force_launches;     /* force-kernel launches timed                     */
+    double  force_ms;           /* summed force-kernel time (HIP events)           */
+    double  tree_ms;            /* summed tree-build time (Barnes-Hut only)        */
+    double  step_ms;            /* wall time of the call on the device (events)   */
+    uint64_t interactions;      /* pair or node interactions evaluated             */
+    int32_t i_per_lane;         /* direct kernel launch shape in use               */
+    int32_t block;
+    int32_t split;
+    int32_t reserved[5];
+} nbody_timing;
+
+/* Turn per-kernel HIP-event timing of nbody_step on or off (default off). */
+nbody_status nbody_set_timing(nbody_sim* sim, int32_t enable);
+
+/* Timing of the last nbody_step call; synchronises the handle. */
+nbody_status nbody_get_timing(nbody_sim* sim, nbody_timing* out);
+
+/* ------------------------------------------------------------------------ */
+/* Names, errors, version                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* NBodySimGetName (src/Sim/INBodySim.cpp:29-39). */
+const char* nbody_method_name(int32_t method);
+const char* nbody_status_string(nbody_status status);
+const char* nbody_last_error(void);
+int32_t     nbody_abi_version(void);
+/* Number of HIP devices visible (0 when none); never fails. */
+int32_t     nbody_device_count(void);
+
+/* ------------------------------------------------------------------------ */
+/* Initial conditions (host, counter-based RNG: particle i's draws depend   */
+/* only on (seed, i), never on the order of generation)                     */
+/* ------------------------------------------------------------------------ */
+
+/* Plummer sphere of scale a and total mass M, isotropic velocities in
+ * virial equilibrium, recentred on the centre of mass.  Outputs pos[3n],
+ * vel[3n], mass[n] (equal masses M/n). */
+nbody_status nbody_ic_plummer(uint64_t seed, size_t n, double a, double total_mass, double G,
+                              float* pos, float* vel, double* mass);
+
+/* Exponential disc (scale length Rd, sech^2 height z0) holding
+ * (1 - bulge_fraction) of the mass on circular orbits, plus a Hernquist
+ * bulge (scale ab, truncated at 50 ab) holding bulge_fraction with
+ * isotropic Gaussian velocities.  Recentred on the centre of mass. */
+nbody_status nbody_ic_disc_bulge(uint64_t seed, size_t n, double Rd, double z0,
+                                 double bulge_fraction, double ab, double total_mass, double G,
+                                 float* pos, float* vel, double* mass);
+
+/* Uniform cube [-half, half)^3 with zero velocities and masses uniform in
+ * [mass_lo, mass_hi). */
+nbody_status nbody_ic_uniform(uint64_t seed, size_t n, double half, double mass_lo, double mass_hi,
+                              float* pos, float* vel, double* mass);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* NBODY_H_ */

@@ -1,0 +1,444 @@
+"""Host-side mirror of the reference's simulator interface over ``libnbody.so``.
+
+The reference exposes its gravity step as ``INBodySim`` with ``Init``,
+``Update`` and ``RenderDebug`` (src/Sim/INBodySim.hpp:19-25), created by
+``CreateNBodySim`` and named by ``NBodySimGetName`` (src/Sim/INBodySim.cpp:7-39).
+This module offers the same surface in Python, calling the HIP library through
+its C ABI (``include/nbody.h``) with ctypes:
+
+* ``ENBodySim`` - the reference's selector (BruteForceCPU, BruteForceGPU,
+  BarnesHut); both brute-force entries run the device direct sum.
+* ``create_nbody_sim(kind, config=None)`` / ``nbody_sim_get_name(kind)``.
+* ``NBodySim.init(particles)`` keeps the caller's record array, like the
+  reference keeps a pointer to the caller's ``std::vector<Particle>``, and
+  ``NBodySim.update(dt)`` advances one step and writes positions, velocities
+  and forces back into it (SimulationState.cpp:52-60).
+* ``PARTICLE_DTYPE`` - the reference's 104-byte ``Particle`` record
+  (src/Render/Misc/Particle.hpp:8-18), also the ``.nbody`` file record.
+
+The library must be built (``make -C procedural-universe_amd``); there is no
+fallback: every call goes to the device code or raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnbody.so")
+
+PARTICLE_DTYPE = np.dtype(
+    [
+        ("position", "<f4", (3,)),
+        ("colour", "<f4", (4,)),
+        ("original_colour", "<f4", (4,)),
+        ("pad_", "<f4"),
+        ("velocity", "<f8", (3,)),
+        ("forces", "<f8", (3,)),
+        ("mass", "<f8"),
+    ]
+)
+assert PARTICLE_DTYPE.itemsize == 104
+
+# nbody_status
+OK = 0
+STATUS_NAMES = {
+    0: "NBODY_OK",
+    1: "NBODY_ERR_INVALID_ARG",
+    2: "NBODY_ERR_OUT_OF_MEMORY",
+    3: "NBODY_ERR_DEVICE",
+    4: "NBODY_ERR_NOT_INITIALISED",
+    5: "NBODY_ERR_UNSUPPORTED",
+    6: "NBODY_ERR_CAPACITY",
+    7: "NBODY_ERR_IO",
+}
+
+METHOD_DIRECT = 1
+METHOD_BARNES_HUT = 2
+LAW_R = 0
+LAW_P = 1
+ROOT_DYNAMIC = 0
+ROOT_FIXED = 1
+FIELD_POSITION = 1
+FIELD_VELOCITY = 2
+FIELD_FORCES = 4
+
+
+class ENBodySim(enum.IntEnum):
+    """src/Sim/INBodySim.hpp:11-17."""
+
+    BruteForceCPU = 0
+    BruteForceGPU = 1
+    BarnesHut = 2
+    NumSims = 3
+
+
+_SIM_NAMES = {
+    ENBodySim.BruteForceCPU: "Brute Force CPU",
+    ENBodySim.BruteForceGPU: "Brute Force GPU",
+    ENBodySim.BarnesHut: "Barnes-Hut",
+}
+
+
+def nbody_sim_get_name(kind) -> str:
+    """NBodySimGetName (src/Sim/INBodySim.cpp:29-39)."""
+    try:
+        return _SIM_NAMES[ENBodySim(kind)]
+    except (ValueError, KeyError):
+        return "Unknown"
+
+
+class Config(ctypes.Structure):
+    """``nbody_config`` of include/nbody.h."""
+
+    _fields_ = [
+        ("method", ctypes.c_int32),
+        ("law", ctypes.c_int32),
+        ("G", ctypes.c_double),
+        ("softening", ctypes.c_double),
+        ("scale", ctypes.c_double),
+        ("theta", ctypes.c_double),
+        ("root_mode", ctypes.c_int32),
+        ("root_lo", ctypes.c_float * 3),
+        ("root_size", ctypes.c_float),
+        ("device", ctypes.c_int32),
+        ("tune_i_per_lane", ctypes.c_int32),
+        ("tune_block", ctypes.c_int32),
+        ("tune_split", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class Diag(ctypes.Structure):
+    _fields_ = [
+        ("kinetic", ctypes.c_double),
+        ("potential", ctypes.c_double),
+        ("total", ctypes.c_double),
+        ("momentum", ctypes.c_double * 3),
+        ("angular_momentum", ctypes.c_double * 3),
+        ("mass", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {
+            "kinetic": self.kinetic,
+            "potential": self.potential,
+            "total": self.total,
+            "momentum": list(self.momentum),
+            "angular_momentum": list(self.angular_momentum),
+            "mass": self.mass,
+        }
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_int32),
+        ("force_launches", ctypes.c_int32),
+        ("force_ms", ctypes.c_double),
+        ("tree_ms", ctypes.c_double),
+        ("step_ms", ctypes.c_double),
+        ("interactions", ctypes.c_uint64),
+        ("i_per_lane", ctypes.c_int32),
+        ("block", ctypes.c_int32),
+        ("split", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
+    ]
+
+
+class NBodyError(RuntimeError):
+    def __init__(self, func: str, status: int, message: str):
+        self.status = status
+        super().__init__(f"{func}: {STATUS_NAMES.get(status, status)}: {message}")
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_F32P = ctypes.POINTER(ctypes.c_float)
+_F64P = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "nbody_config_default": (None, [ctypes.POINTER(Config)]),
+    "nbody_config_reference": (None, [ctypes.POINTER(Config), ctypes.c_int32]),
+    "nbody_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(_P)]),
+    "nbody_destroy": (ctypes.c_int, [_P]),
+    "nbody_init": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "nbody_init_soa": (ctypes.c_int, [_P, _F32P, _F32P, _F64P, ctypes.c_size_t]),
+    "nbody_step": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_int32]),
+    "nbody_synchronize": (ctypes.c_int, [_P]),
+    "nbody_read": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.c_uint32]),
+    "nbody_read_soa": (ctypes.c_int, [_P, _F32P, _F32P, _F32P, ctypes.c_size_t]),
+    "nbody_compute_accelerations": (ctypes.c_int, [_P, _F32P, ctypes.c_size_t]),
+    "nbody_size": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    "nbody_diagnostics": (ctypes.c_int, [_P, ctypes.POINTER(Diag)]),
+    "nbody_set_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "nbody_get_timing": (ctypes.c_int, [_P, ctypes.POINTER(Timing)]),
+    "nbody_method_name": (ctypes.c_char_p, [ctypes.c_int32]),
+    "nbody_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "nbody_last_error": (ctypes.c_char_p, []),
+    "nbody_abi_version": (ctypes.c_int32, []),
+    "nbody_device_count": (ctypes.c_int32, []),
+    "nbody_ic_plummer": (
+        ctypes.c_int,
+        [ctypes.c_uint64, ctypes.c_size_t, ctypes.c_double, ctypes.c_double, ctypes.c_double, _F32P, _F32P, _F64P],
+    ),
+    "nbody_ic_disc_bulge": (
+        ctypes.c_int,
+        [
+            ctypes.c_uint64,
+            ctypes.c_size_t,
+            ctypes.c_double,
+            ctypes.c_double,
+            ctypes.c_double,
+            ctypes.c_double,
+            ctypes.c_double,
+            ctypes.c_double,
+            _F32P,
+            _F32P,
+            _F64P,
+        ],
+    ),
+    "nbody_ic_uniform": (
+        ctypes.c_int,
+        [ctypes.c_uint64, ctypes.c_size_t, ctypes.c_double, ctypes.c_double, ctypes.c_double, _F32P, _F32P, _F64P],
+    ),
+}
+
+
+def load_library(path: Optional[str] = None) -> ctypes.CDLL:
+    """Load libnbody.so (raises OSError when it is not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise OSError(f"{p} is not built; run `make -C {_HERE}` or __graft_entry__.build()")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(func: str, status: int):
+    if status != OK:
+        msg = load_library().nbody_last_error()
+        raise NBodyError(func, status, msg.decode() if msg else "")
+
+
+def _f32(a):
+    return a.ctypes.data_as(_F32P)
+
+
+def _f64(a):
+    return a.ctypes.data_as(_F64P)
+
+
+def device_count() -> int:
+    return int(load_library().nbody_device_count())
+
+
+def default_config(**overrides) -> Config:
+    c = Config()
+    load_library().nbody_config_default(ctypes.byref(c))
+    return _apply(c, overrides)
+
+
+def reference_config(method: int = METHOD_DIRECT, **overrides) -> Config:
+    """The reference's constants (src/Sim/Physics.hpp:9-16, Octree.cpp:5, BarnesHut.cpp:14-19)."""
+    c = Config()
+    load_library().nbody_config_reference(ctypes.byref(c), int(method))
+    return _apply(c, overrides)
+
+
+def _apply(c: Config, overrides) -> Config:
+    for k, v in overrides.items():
+        if k == "root_lo":
+            for i in range(3):
+                c.root_lo[i] = float(v[i])
+        else:
+            setattr(c, k, v)
+    return c
+
+
+class NBodySim:
+    """One device simulation: the INBodySim contract (src/Sim/INBodySim.hpp:19-25)."""
+
+    def __init__(self, config: Config):
+        lib = load_library()
+        self._lib = lib
+        self.config = config
+        h = _P()
+        _check("nbody_create", lib.nbody_create(ctypes.byref(config), ctypes.byref(h)))
+        self._h = h
+        self._particles = None
+        self.n = 0
+        # BruteForceCPU/GPU leave Forces zeroed after Update (BruteForceCPU.cpp:72,
+        # Update.csh:35); BarnesHut leaves the last forces (BarnesHut.cpp:75,108).
+        self.zero_forces_after_update = config.method == METHOD_DIRECT
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nbody_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- INBodySim --------------------------------------------------------
+    def init(self, particles: np.ndarray):
+        """INBodySim::Init: adopt the caller's record array (updated in place by update())."""
+        if particles.dtype != PARTICLE_DTYPE or not particles.flags["C_CONTIGUOUS"]:
+            raise TypeError("particles must be a C-contiguous array of PARTICLE_DTYPE")
+        _check("nbody_init", self._lib.nbody_init(self._h, particles.ctypes.data, particles.size))
+        self._particles = particles
+        self.n = particles.size
+
+    def update(self, dt: float):
+        """INBodySim::Update: one synchronous step, written back into the adopted records."""
+        if self._particles is None:
+            raise RuntimeError("update() before init()")
+        self.step(dt, 1)
+        p = self._particles
+        _check(
+            "nbody_read",
+            self._lib.nbody_read(self._h, p.ctypes.data, p.size, FIELD_POSITION | FIELD_VELOCITY | FIELD_FORCES),
+        )
+        if self.zero_forces_after_update:
+            p["forces"] = 0.0
+
+    def render_debug(self, view=None, proj=None):
+        """INBodySim::RenderDebug has a do-nothing default (INBodySim.hpp:23)."""
+        return None
+
+    # -- structure-of-arrays API ----------------------------------------
+    def init_soa(self, pos: np.ndarray, vel: Optional[np.ndarray], mass: np.ndarray):
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        mass = np.ascontiguousarray(mass, dtype=np.float64).reshape(n)
+        velp = None
+        if vel is not None:
+            vel = np.ascontiguousarray(vel, dtype=np.float32).reshape(n, 3)
+            velp = _f32(vel)
+        _check("nbody_init_soa", self._lib.nbody_init_soa(self._h, _f32(pos), velp, _f64(mass), n))
+        self._particles = None
+        self.n = n
+
+    def step(self, dt: float, nsteps: int = 1):
+        _check("nbody_step", self._lib.nbody_step(self._h, float(dt), int(nsteps)))
+
+    def synchronize(self):
+        _check("nbody_synchronize", self._lib.nbody_synchronize(self._h))
+
+    def read_soa(self):
+        n = self.n
+        pos = np.empty((n, 3), np.float32)
+        vel = np.empty((n, 3), np.float32)
+        acc = np.empty((n, 3), np.float32)
+        _check("nbody_read_soa", self._lib.nbody_read_soa(self._h, _f32(pos), _f32(vel), _f32(acc), n))
+        return pos, vel, acc
+
+    def read_particles(self, particles: np.ndarray, fields: int = 7):
+        _check("nbody_read", self._lib.nbody_read(self._h, particles.ctypes.data, particles.size, fields))
+
+    def accelerations(self) -> np.ndarray:
+        acc = np.empty((self.n, 3), np.float32)
+        _check("nbody_compute_accelerations", self._lib.nbody_compute_accelerations(self._h, _f32(acc), self.n))
+        return acc
+
+    def diagnostics(self) -> dict:
+        d = Diag()
+        _check("nbody_diagnostics", self._lib.nbody_diagnostics(self._h, ctypes.byref(d)))
+        return d.as_dict()
+
+    def set_timing(self, enable: bool = True):
+        _check("nbody_set_timing", self._lib.nbody_set_timing(self._h, 1 if enable else 0))
+
+    def timing(self) -> Timing:
+        t = Timing()
+        _check("nbody_get_timing", self._lib.nbody_get_timing(self._h, ctypes.byref(t)))
+        return t
+
+
+def create_nbody_sim(kind, config: Optional[Config] = None) -> NBodySim:
+    """CreateNBodySim (src/Sim/INBodySim.cpp:7-27): reference constants unless a config is given."""
+    kind = ENBodySim(kind)
+    if kind == ENBodySim.NumSims:
+        raise ValueError("NumSims is not a simulator")
+    method = METHOD_BARNES_HUT if kind == ENBodySim.BarnesHut else METHOD_DIRECT
+    if config is None:
+        config = reference_config(method)
+    else:
+        config.method = method
+    return NBodySim(config)
+
+
+# -- initial conditions ----------------------------------------------------
+def _ic_out(n):
+    return np.empty((n, 3), np.float32), np.empty((n, 3), np.float32), np.empty(n, np.float64)
+
+
+def ic_plummer(n: int, seed: int, a: float = 1.0, total_mass: float = 1.0, G: float = 1.0):
+    pos, vel, mass = _ic_out(n)
+    _check(
+        "nbody_ic_plummer",
+        load_library().nbody_ic_plummer(seed, n, a, total_mass, G, _f32(pos), _f32(vel), _f64(mass)),
+    )
+    return pos, vel, mass
+
+
+def ic_disc_bulge(
+    n: int,
+    seed: int,
+    Rd: float = 3.0,
+    z0: float = 0.3,
+    bulge_fraction: float = 0.2,
+    ab: float = 0.6,
+    total_mass: float = 1.0,
+    G: float = 1.0,
+):
+    pos, vel, mass = _ic_out(n)
+    _check(
+        "nbody_ic_disc_bulge",
+        load_library().nbody_ic_disc_bulge(
+            seed, n, Rd, z0, bulge_fraction, ab, total_mass, G, _f32(pos), _f32(vel), _f64(mass)
+        ),
+    )
+    return pos, vel, mass
+
+
+def ic_uniform(n: int, seed: int, half: float = 500.0, mass_lo: float = 1e20, mass_hi: float = 1e30):
+    pos, vel, mass = _ic_out(n)
+    _check(
+        "nbody_ic_uniform",
+        load_library().nbody_ic_uniform(seed, n, half, mass_lo, mass_hi, _f32(pos), _f32(vel), _f64(mass)),
+    )
+    return pos, vel, mass
+
+
+def particles_from_soa(pos, vel, mass) -> np.ndarray:
+    """Pack SoA arrays into reference-layout records (colours zero)."""
+    n = len(mass)
+    p = np.zeros(n, PARTICLE_DTYPE)
+    p["position"] = pos
+    p["velocity"] = vel
+    p["mass"] = mass
+    return p

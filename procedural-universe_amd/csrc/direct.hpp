@@ -1,0 +1,56 @@
+// Direct-sum force pass: launch interface shared by the single-system
+// simulator, the batch executor and the autotuner.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace nbody {
+
+// j-particles staged in LDS per tile; particle arrays are padded to a
+// multiple of this with massless dummies at the origin.
+constexpr int kTile = 256;
+
+// Launch shape of the direct kernel: targets per lane, threads per
+// workgroup, and number of j-chunks each target's sum is split into.
+struct DirectShape {
+    int i_per_lane;  // 1, 2, 4 or 8
+    int block;       // 64, 128 or 256
+    int split;       // >= 1; > 1 adds a fixed-order reduction pass
+};
+
+struct DirectLaunch {
+    const float4* src;  // n_pad records (x, y, z, G*m): positions at step start
+    float4* part;       // split > 1 or no kick: partial sums [split][n_pad]
+    float4* pos_out;    // kick: positions after the step (other buffer)
+    float4* vel;        // kick: velocities, updated in place
+    float4* acc;        // accelerations of this evaluation (n_pad)
+    int n;              // real particles
+    int n_pad;          // padded count, multiple of kTile
+    int law;            // nbody_law
+    float soft;         // S (law R) or eps^2 (law P)
+    float tiny;         // law R: added to r2 (r2 + S)^2; keeps G m_j / sqrt(tiny) finite
+    float dt;           // kick step
+    float dts;          // drift factor dt / scale
+};
+
+bool direct_shape_valid(const DirectShape& s);
+
+// Smallest `tiny` (>= FLT_MIN) for which max_gm / sqrt(tiny) stays below FLT_MAX / 2.
+float direct_tiny_for(double max_gm);
+
+// Effective number of j-chunks for a shape at this size (never an empty chunk).
+int direct_effective_split(const DirectShape& s, int n_pad);
+
+// Heuristic shape: enough workgroups to cover all CUs several times.
+DirectShape direct_default_shape(int n_pad);
+
+// One force evaluation.  kick = true advances positions/velocities (kick then
+// drift) and writes acc; kick = false only writes acc.  With an effective
+// split of 1 and kick the whole step is one fused kernel; otherwise the force
+// kernel writes partial sums and a second kernel adds them in chunk order.
+hipError_t direct_step(const DirectLaunch& L, const DirectShape& s, bool kick, hipStream_t stream,
+                       hipEvent_t ev_force_begin, hipEvent_t ev_force_end);
+
+}  // namespace nbody
