@@ -212,6 +212,24 @@ nbody_status nbody_set_timing(nbody_sim* sim, int32_t enable);
 nbody_status nbody_get_timing(nbody_sim* sim, nbody_timing* out);
 
 /* ------------------------------------------------------------------------ */
+/* Barnes-Hut introspection                                                  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct nbody_bh_stats {
+    uint64_t interactions;    /* cell + particle interactions since the last nbody_step began   */
+    uint32_t outside_root;    /* particles clamped into a fixed root (the reference drops them) */
+    uint32_t stack_overflow;  /* nonzero if a traversal stack overflowed (forces invalid)        */
+    int32_t  reserved[4];
+} nbody_bh_stats;
+
+/* Counters of the Barnes-Hut steps since the last nbody_step began. */
+nbody_status nbody_bh_stats_get(nbody_sim* sim, nbody_bh_stats* out);
+
+/* Sorted 63-bit Morton keys and the particle order (stable) of the last tree
+ * build, for checking against a CPU computation.  Either pointer may be NULL. */
+nbody_status nbody_bh_tree_order(nbody_sim* sim, uint64_t* keys, uint32_t* order, size_t n);
+
+/* ------------------------------------------------------------------------ */
 /* Names, errors, version                                                    */
 /* ------------------------------------------------------------------------ */
 

@@ -130,6 +130,7 @@ struct Node {
     int32_t p = -1;       // P (particle index)
     double mass = 0.0;    // TotalMass
     V3 com{0.f, 0.f, 0.f};  // CentreOfMass
+    double mx = 0.0, my = 0.0, mz = 0.0;  // FP64 mass-weighted position (com_fp64 mode)
     int32_t child[8];
 };
 
@@ -137,6 +138,7 @@ struct Tree {
     std::vector<Node> nodes;
     const float* pos;
     const double* mass;
+    bool com_fp64 = false;  // FP64 centre of mass instead of the reference's FP32 one
     int64_t dropped = 0;
     int max_depth = 0;
 
@@ -217,8 +219,30 @@ struct Tree {
     }
 
     // Octree::CalculateMass (Octree.cpp:86-105): FP64 mass, FP32 centre of mass.
+    // With com_fp64 the mass-weighted position is summed in FP64 and divided
+    // once (the device tree's arithmetic); the cells and the masses are the same.
     void calculate_mass(int id) {
         Node& nd = nodes[id];
+        if (com_fp64) {
+            if (nd.num == 1) {
+                const V3 q = load(pos, static_cast<size_t>(nd.p));
+                nd.mass = mass[nd.p];
+                nd.mx = nd.mass * q.x, nd.my = nd.mass * q.y, nd.mz = nd.mass * q.z;
+                nd.com = q;
+            } else if (!nd.leaf) {
+                double M = 0, X = 0, Y = 0, Z = 0;
+                for (int k = 0; k < 8; ++k) {
+                    const int ch = nodes[id].child[k];
+                    calculate_mass(ch);
+                    const Node& c = nodes[ch];
+                    M += c.mass, X += c.mx, Y += c.my, Z += c.mz;
+                }
+                Node& n2 = nodes[id];
+                n2.mass = M, n2.mx = X, n2.my = Y, n2.mz = Z;
+                if (M > 0.0) n2.com = {static_cast<float>(X / M), static_cast<float>(Y / M), static_cast<float>(Z / M)};
+            }
+            return;
+        }
         if (nd.num == 1) {
             nd.com = load(pos, static_cast<size_t>(nd.p));
             nd.mass = mass[nd.p];
@@ -273,10 +297,12 @@ struct Tree {
     }
 };
 
-std::unique_ptr<Tree> build_tree(size_t n, const float* pos, const double* mass, const float* lo, float size) {
+std::unique_ptr<Tree> build_tree(size_t n, const float* pos, const double* mass, const float* lo, float size,
+                                 bool com_fp64 = false) {
     auto t = std::make_unique<Tree>();
     t->pos = pos;
     t->mass = mass;
+    t->com_fp64 = com_fp64;
     t->nodes.reserve(4 * n + 16);
     Cube root;
     for (int a = 0; a < 3; ++a) {
@@ -435,7 +461,13 @@ void oracle_fp64_energy(int law, size_t n, const float* pos, const float* vel, c
 
 void oracle_bh_forces(size_t n, const float* pos, const double* mass, double G, double S, double theta,
                       const float* root_lo, float root_size, int workers, double* out_forces, int64_t* stats) {
-    auto t = build_tree(n, pos, mass, root_lo, root_size);
+    oracle_bh_forces_ex(n, pos, mass, G, S, theta, root_lo, root_size, 0, workers, out_forces, stats);
+}
+
+void oracle_bh_forces_ex(size_t n, const float* pos, const double* mass, double G, double S, double theta,
+                         const float* root_lo, float root_size, int com_fp64, int workers, double* out_forces,
+                         int64_t* stats) {
+    auto t = build_tree(n, pos, mass, root_lo, root_size, com_fp64 != 0);
     parallel_for(n, workers, [&](size_t i) {
         double f[3] = {0, 0, 0};
         t->calculate_force(0, static_cast<int32_t>(i), G, S, theta, f);

@@ -57,6 +57,13 @@ void oracle_fp64_energy(int law, size_t n, const float* pos, const float* vel, c
 void oracle_bh_forces(size_t n, const float* pos, const double* mass, double G, double S, double theta,
                       const float* root_lo, float root_size, int workers, double* out_forces, int64_t* stats);
 
+/* As oracle_bh_forces; com_fp64 != 0 sums the cells' mass-weighted positions
+ * in FP64 (the device tree's arithmetic) instead of the reference's FP32
+ * accumulation (Octree.cpp:99,103), with the same cells, masses and opening test. */
+void oracle_bh_forces_ex(size_t n, const float* pos, const double* mass, double G, double S, double theta,
+                         const float* root_lo, float root_size, int com_fp64, int workers, double* out_forces,
+                         int64_t* stats);
+
 /* BarnesHut::Update (src/Sim/BarnesHut.cpp:44-96) repeated nsteps times:
  * rebuild, forces, v += F/m*dt, x += (float)(v*dt/scale). */
 void oracle_bh_step(size_t n, float* pos, double* vel, const double* mass, double G, double S, double scale,

@@ -149,6 +149,15 @@ class Timing(ctypes.Structure):
     ]
 
 
+class BHStats(ctypes.Structure):
+    _fields_ = [
+        ("interactions", ctypes.c_uint64),
+        ("outside_root", ctypes.c_uint32),
+        ("stack_overflow", ctypes.c_uint32),
+        ("reserved", ctypes.c_int32 * 4),
+    ]
+
+
 class NBodyError(RuntimeError):
     def __init__(self, func: str, status: int, message: str):
         self.status = status
@@ -178,6 +187,11 @@ _SIGNATURES = {
     "nbody_diagnostics": (ctypes.c_int, [_P, ctypes.POINTER(Diag)]),
     "nbody_set_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
     "nbody_get_timing": (ctypes.c_int, [_P, ctypes.POINTER(Timing)]),
+    "nbody_bh_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(BHStats)]),
+    "nbody_bh_tree_order": (
+        ctypes.c_int,
+        [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32), ctypes.c_size_t],
+    ),
     "nbody_method_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "nbody_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "nbody_last_error": (ctypes.c_char_p, []),
@@ -368,6 +382,25 @@ class NBodySim:
         d = Diag()
         _check("nbody_diagnostics", self._lib.nbody_diagnostics(self._h, ctypes.byref(d)))
         return d.as_dict()
+
+    def bh_stats(self) -> dict:
+        st = BHStats()
+        _check("nbody_bh_stats_get", self._lib.nbody_bh_stats_get(self._h, ctypes.byref(st)))
+        return {"interactions": int(st.interactions), "outside_root": int(st.outside_root),
+                "stack_overflow": int(st.stack_overflow)}
+
+    def bh_tree_order(self):
+        """Sorted Morton keys and particle order of the last Barnes-Hut tree build."""
+        keys = np.empty(self.n, np.uint64)
+        order = np.empty(self.n, np.uint32)
+        _check(
+            "nbody_bh_tree_order",
+            self._lib.nbody_bh_tree_order(
+                self._h, keys.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                order.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), self.n
+            ),
+        )
+        return keys, order
 
     def set_timing(self, enable: bool = True):
         _check("nbody_set_timing", self._lib.nbody_set_timing(self._h, 1 if enable else 0))

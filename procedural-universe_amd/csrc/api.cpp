@@ -64,6 +64,7 @@ nbody_status validate_config(const nbody_config* c) {
                      std::isfinite(c->root_lo[1]) && std::isfinite(c->root_lo[2]),
                  NBODY_ERR_INVALID_ARG, "fixed root needs a finite lower corner and size > 0");
     }
+    if (c->method == NBODY_METHOD_BARNES_HUT && c->root_mode == NBODY_ROOT_FIXED) NB_TRY(bh_fixed_root(c, nullptr));
     if (c->tune_i_per_lane || c->tune_block || c->tune_split) {
         DirectShape s{c->tune_i_per_lane ? c->tune_i_per_lane : 4, c->tune_block ? c->tune_block : 256,
                       c->tune_split ? c->tune_split : 1};
@@ -279,6 +280,7 @@ nbody_status nbody_destroy(nbody_sim* s) {
     if (s->bh) bh_destroy(s->bh);
     release_state(s);
     for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
+    for (hipEvent_t e : s->tree_ms_pending) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(s->stream);
     delete s;
     return NBODY_OK;
@@ -325,7 +327,12 @@ nbody_status nbody_step(nbody_sim* s, float dt, int32_t nsteps) {
     NB_HIP(hipSetDevice(s->cfg.device));
     s->last = nbody_timing{};
     s->step_interactions = 0;
-    s->tree_ms_pending.clear();
+    if (!s->tree_ms_pending.empty()) {
+        NB_HIP(hipStreamSynchronize(s->stream));
+        for (hipEvent_t e : s->tree_ms_pending) (void)hipEventDestroy(e);
+        s->tree_ms_pending.clear();
+    }
+    if (s->bh) NB_TRY(bh_reset_counters(s->bh, s->stream));
     if (s->n == 0) return NBODY_OK;
     if (s->timing) NB_TRY(ensure_events(s, static_cast<size_t>(nsteps) + 1));
     if (s->timing) NB_HIP(hipEventRecord(s->events[0], s->stream));
@@ -471,6 +478,11 @@ nbody_status nbody_get_timing(nbody_sim* s, nbody_timing* out) {
     NB_HIP(hipSetDevice(s->cfg.device));
     NB_HIP(hipStreamSynchronize(s->stream));
     *out = s->last;
+    if (s->bh && s->n >= 2) {
+        unsigned long long it = 0;
+        NB_TRY(bh_read_counters(s->bh, s->stream, &it, nullptr, nullptr));
+        out->interactions = it;
+    }
     if (s->timed_steps > 0) {
         float ms = 0;
         NB_HIP(hipEventElapsedTime(&ms, s->events[0], s->events[1]));
@@ -488,6 +500,30 @@ nbody_status nbody_get_timing(nbody_sim* s, nbody_timing* out) {
         }
     }
     return NBODY_OK;
+}
+
+nbody_status nbody_bh_stats_get(nbody_sim* s, nbody_bh_stats* out) {
+    clear_error();
+    NB_CHECK(s && out, NBODY_ERR_INVALID_ARG, "null argument");
+    NB_CHECK(s->bh, NBODY_ERR_UNSUPPORTED, "not a Barnes-Hut simulation");
+    std::memset(out, 0, sizeof *out);
+    NB_HIP(hipSetDevice(s->cfg.device));
+    unsigned long long it = 0;
+    unsigned outside = 0, overflow = 0;
+    NB_TRY(bh_read_counters(s->bh, s->stream, &it, &outside, &overflow));
+    out->interactions = it;
+    out->outside_root = outside;
+    out->stack_overflow = overflow;
+    return NBODY_OK;
+}
+
+nbody_status nbody_bh_tree_order(nbody_sim* s, uint64_t* keys, uint32_t* order, size_t n) {
+    clear_error();
+    NB_CHECK(s, NBODY_ERR_INVALID_ARG, "sim is null");
+    NB_CHECK(s->bh, NBODY_ERR_UNSUPPORTED, "not a Barnes-Hut simulation");
+    NB_CHECK(s->initialised && n == s->n, NBODY_ERR_INVALID_ARG, "n = %zu does not match the state's %zu", n, s->n);
+    NB_HIP(hipSetDevice(s->cfg.device));
+    return bh_export_order(s, keys, order, n);
 }
 
 const char* nbody_method_name(int32_t method) {

@@ -1,15 +1,958 @@
-// Barnes-Hut: placeholder until the device tree lands.
+// Barnes-Hut gravity on gfx950.
+//
+// Replaces the reference's BarnesHut::Update / Exec (src/Sim/BarnesHut.cpp:44-110)
+// and its pointer octree (src/Sim/Octree.cpp:16-145), which is rebuilt serially
+// by per-particle insertion every step.  Here every stage is a device pass:
+//
+//  1. root cell      fixed cube (the reference's [-4000, 4000)^3) or the
+//                    bounding box grown to a power-of-two cube (two-level
+//                    min/max reduction);
+//  2. Morton keys    63 bits, 21 levels, x in the lowest bit of each 3-bit
+//                    group, which is the reference's child order x + 2y + 4z
+//                    (Octree.cpp:25-41).  The cell index along an axis is
+//                    floor((x - lo) / cell) computed exactly: x is scaled by a
+//                    power of two (exact), floored, and divided by the odd part
+//                    of the root size in integers, so keys equal the CPU's bit
+//                    for bit and the cells are the reference's half-open cubes;
+//  3. sort           least-significant-digit radix sort of (key, index), eight
+//                    8-bit passes of histogram / scan / scatter; the scatter
+//                    ranks equal digits with wave ballots in LDS, so the order
+//                    is stable and equals std::stable_sort of the keys;
+//  4. radix tree     binary radix tree over the sorted keys (Karras 2012);
+//                    a node whose common prefix reaches a deeper 3-bit group
+//                    than its parent's is an octree cell ("kept"), the others
+//                    are the binary splits inside one cell; identical keys
+//                    form a level-21 bucket;
+//  5. mass           FP64 mass and mass-weighted position summed bottom-up
+//                    (the second child to finish adds both, in a fixed order);
+//  6. traversal      one lane per particle in Morton order; a wave walks the
+//                    octree with one stack in LDS of (cell, lanes) entries.
+//                    Each lane makes the reference's own decision at each cell
+//                    (Octree::CalculateForce, Octree.cpp:107-145): approximate
+//                    with the cell's monopole iff side / |p - COM| < theta,
+//                    otherwise open it; a cell is pushed with the mask of the
+//                    lanes that opened it, so the wave shares the walk while
+//                    every particle gets exactly its own set of interactions.
+//                    Single particles are exact pair forces (skipping self);
+//                    an approximated cell that contains p includes p's own mass,
+//                    as in the reference.  Kick and drift are fused.
+//
+// A compressed octree gives the reference's forces: along a chain of cells
+// with one non-empty child the mass and centre of mass do not change and the
+// side shrinks, so the reference approximates somewhere on the chain iff it
+// approximates at the chain's deepest cell, which is the cell kept here.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
 #include "bh.hpp"
 #include "common.hpp"
+#include "direct.hpp"
 #include "sim.hpp"
 
 namespace nbody {
-struct BHState {};
-nbody_status bh_create(nbody_sim*) {
-    set_error("Barnes-Hut is not built yet");
-    return NBODY_ERR_UNSUPPORTED;
+
+namespace {
+
+constexpr int kLevels = 21;
+constexpr int kBucketLevel = kLevels;
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kScanThreads = 1024;
+constexpr int kTravThreads = 256;
+constexpr int kTravWaves = kTravThreads / 64;
+constexpr int kStackCap = 176;  // > 7 * 22 + 8 pending cells per wave
+constexpr int kEmpty = -1;
+
+// Child encoding: >= 0 internal node; <= -2 particle at sorted position (-c - 2).
+__host__ __device__ inline int enc_leaf(int k) { return -k - 2; }
+__host__ __device__ inline int dec_leaf(int c) { return -c - 2; }
+
+struct RootParams {
+    float lo[3];
+    float size;    // edge length of the root cube
+    float scale2;  // 2^(21 - p) with size = odd * 2^p
+    int odd;       // odd part of the size (1 for a power-of-two root)
+    int pad;
+    long long base[3];  // floor_div(floor(lo * scale2), odd)
+};
+
+__host__ __device__ inline long long floor_div_ll(long long a, long long b) {
+    long long q = a / b;
+    if ((a % b != 0) && (a < 0)) --q;
+    return q;
 }
-void bh_destroy(BHState* b) { delete b; }
-nbody_status bh_resize(BHState*, size_t, nbody_sim*) { return NBODY_OK; }
-nbody_status bh_step(nbody_sim*, float, bool, hipEvent_t, hipEvent_t) { return NBODY_ERR_UNSUPPORTED; }
+
+__device__ inline unsigned long long spread3(unsigned long long v) {
+    v &= 0x1fffffULL;
+    v = (v | (v << 32)) & 0x1f00000000ffffULL;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
+    v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
+    v = (v | (v << 2)) & 0x1249249249249249ULL;
+    return v;
+}
+
+// Exact floor((x - lo) / cell) for a root corner that is a multiple of the cell.
+__device__ inline long long cell_index(float x, const RootParams& R, int a) {
+    float X = x * R.scale2;  // power-of-two scaling: exact
+    X = fmaxf(fminf(X, 0x1p40f), -0x1p40f);
+    long long k = static_cast<long long>(floorf(X));
+    if (X == 0.f && x < 0.f) k = -1;
+    return floor_div_ll(k, R.odd) - R.base[a];
+}
+
+// ---- 1. root -------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void bounds_kernel(const float4* __restrict__ pos, int n,
+                                                     float* __restrict__ partial) {
+    __shared__ float red[6][256];
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const float4 p = pos[i];
+        mn[0] = fminf(mn[0], p.x), mn[1] = fminf(mn[1], p.y), mn[2] = fminf(mn[2], p.z);
+        mx[0] = fmaxf(mx[0], p.x), mx[1] = fmaxf(mx[1], p.y), mx[2] = fmaxf(mx[2], p.z);
+    }
+    for (int a = 0; a < 3; ++a) {
+        red[a][threadIdx.x] = mn[a];
+        red[3 + a][threadIdx.x] = mx[a];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int a = 0; a < 3; ++a) {
+                red[a][threadIdx.x] = fminf(red[a][threadIdx.x], red[a][threadIdx.x + w]);
+                red[3 + a][threadIdx.x] = fmaxf(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) partial[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Power-of-two cube over the bounding box with a corner on its level-21 grid
+// (same arithmetic as the CPU's oracle_dynamic_root).
+__global__ void dynamic_root_kernel(const float* __restrict__ partial, int nblocks, RootParams* __restrict__ R) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int b = 0; b < nblocks; ++b)
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = fminf(mn[a], partial[b * 6 + a]);
+            mx[a] = fmaxf(mx[a], partial[b * 6 + 3 + a]);
+        }
+    float ext = 0.f;
+    for (int a = 0; a < 3; ++a) ext = fmaxf(ext, mx[a] - mn[a]);
+    int e;
+    frexpf(fmaxf(ext, 0x1p-100f), &e);
+    float size = ldexpf(1.0f, e);
+    RootParams out{};
+    for (int tries = 0; tries < 4; ++tries) {
+        const float cell = ldexpf(size, -kLevels);
+        bool fits = true;
+        for (int a = 0; a < 3; ++a) {
+            const long long k0 = static_cast<long long>(floorf(mn[a] / cell));
+            const long long k1 = static_cast<long long>(floorf(mx[a] / cell));
+            out.lo[a] = floorf(mn[a] / cell) * cell;
+            out.base[a] = k0;
+            if (k1 - k0 >= (1LL << kLevels)) fits = false;
+        }
+        if (fits) break;
+        size *= 2.f;
+    }
+    out.size = size;
+    out.scale2 = ldexpf(1.0f, kLevels) / size;
+    out.odd = 1;
+    *R = out;
+}
+
+// ---- 2. keys ---------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void keys_kernel(const float4* __restrict__ pos, int n,
+                                                   const RootParams* __restrict__ Rp,
+                                                   unsigned long long* __restrict__ keys,
+                                                   unsigned* __restrict__ idx, unsigned* __restrict__ outside) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const RootParams R = *Rp;
+    const float4 p = pos[i];
+    const float c[3] = {p.x, p.y, p.z};
+    unsigned long long key = 0;
+    bool out = false;
+    for (int a = 0; a < 3; ++a) {
+        long long k = cell_index(c[a], R, a);
+        if (k < 0) k = 0, out = true;
+        if (k > 0x1fffff) k = 0x1fffff, out = true;
+        key |= spread3(static_cast<unsigned long long>(k)) << a;
+    }
+    keys[i] = key;
+    idx[i] = static_cast<unsigned>(i);
+    if (out) atomicAdd(outside, 1u);
+}
+
+// ---- 3. radix sort -----------------------------------------------------------
+
+__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const unsigned long long* __restrict__ keys, int n,
+                                                                  int shift, unsigned* __restrict__ hist,
+                                                                  int nblocks) {
+    __shared__ unsigned h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kSortTile;
+    for (int k = 0; k < kSortItems; ++k) {
+        const int i = base + k * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of `total` counters in one workgroup.
+__global__ __launch_bounds__(kScanThreads) void radix_scan_kernel(unsigned* __restrict__ hist, int total) {
+    __shared__ unsigned sums[kScanThreads];
+    const int t = threadIdx.x;
+    const int per = (total + kScanThreads - 1) / kScanThreads;
+    const int b = t * per, e = min(b + per, total);
+    unsigned s = 0;
+    for (int i = b; i < e; ++i) s += hist[i];
+    sums[t] = s;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const unsigned v = t >= off ? sums[t - off] : 0u;
+        __syncthreads();
+        sums[t] += v;
+        __syncthreads();
+    }
+    unsigned run = sums[t] - s;
+    for (int i = b; i < e; ++i) {
+        const unsigned v = hist[i];
+        hist[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
+    const unsigned long long* __restrict__ keys_in, const unsigned* __restrict__ idx_in,
+    unsigned long long* __restrict__ keys_out, unsigned* __restrict__ idx_out, int n, int shift,
+    const unsigned* __restrict__ offsets, int nblocks) {
+    constexpr int W = kSortThreads / 64;
+    __shared__ unsigned cnt[W][256];
+    __shared__ unsigned running[256];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    running[tid] = offsets[tid * nblocks + blockIdx.x];
+    const unsigned long long below = (1ULL << lane) - 1ULL;
+    const int base = blockIdx.x * kSortTile;
+    for (int k = 0; k < kSortItems; ++k) {
+        const int i = base + k * kSortThreads + tid;
+        const bool valid = i < n;
+        const unsigned long long key = valid ? keys_in[i] : 0ULL;
+        const unsigned digit = static_cast<unsigned>((key >> shift) & 255u);
+        unsigned long long peers = __ballot(valid);
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool set = (digit >> bit) & 1u;
+            const unsigned long long b = __ballot(valid && set);
+            peers &= set ? b : ~b;
+        }
+        const unsigned rank = static_cast<unsigned>(__popcll(peers & below));
+        for (int w = 0; w < W; ++w) cnt[w][tid] = 0;
+        __syncthreads();
+        if (valid && rank == 0) cnt[wave][digit] = static_cast<unsigned>(__popcll(peers));
+        __syncthreads();
+        {
+            unsigned run = running[tid];
+            for (int w = 0; w < W; ++w) {
+                const unsigned c = cnt[w][tid];
+                cnt[w][tid] = run;
+                run += c;
+            }
+            running[tid] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            const unsigned dst = cnt[wave][digit] + rank;
+            keys_out[dst] = key;
+            idx_out[dst] = idx_in[i];
+        }
+        __syncthreads();
+    }
+}
+
+// ---- 4. radix tree -------------------------------------------------------------
+
+__device__ inline int delta(const unsigned long long* __restrict__ keys, int n, int a, int b) {
+    if (b < 0 || b >= n) return -1;
+    const unsigned long long ka = keys[a], kb = keys[b];
+    if (ka == kb) return 64 + __clz(static_cast<unsigned>(a ^ b));
+    return __clzll(ka ^ kb);
+}
+
+// Cell level of a binary node with common-prefix length d (keys use 63 bits,
+// the top bit is always 0): the number of whole 3-bit groups in common.
+__device__ inline int level_of(int d) { return d >= 64 ? kBucketLevel : (d - 1) / 3; }
+
+struct TreeArgs {
+    const unsigned long long* keys;  // sorted
+    int n;
+    int* left;         // [n-1] encoded child
+    int* right;        // [n-1] encoded child
+    int* parent;       // [n-1] parent of internal node (-1 at the root)
+    int* leaf_parent;  // [n]
+    int* first;        // [n-1]
+    int* last;         // [n-1]
+    int* level;        // [n-1]
+};
+
+__global__ __launch_bounds__(256) void radix_tree_kernel(TreeArgs T) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int n = T.n;
+    if (i >= n - 1) return;
+    const unsigned long long* keys = T.keys;
+    const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(keys, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + min(d, 0);
+    const int lo = min(i, j), hi = max(i, j);
+    const int lc = (lo == gamma) ? enc_leaf(gamma) : gamma;
+    const int rc = (hi == gamma + 1) ? enc_leaf(gamma + 1) : gamma + 1;
+    T.left[i] = lc;
+    T.right[i] = rc;
+    T.first[i] = lo;
+    T.last[i] = hi;
+    T.level[i] = level_of(dnode);
+    if (lc >= 0) T.parent[lc] = i; else T.leaf_parent[dec_leaf(lc)] = i;
+    if (rc >= 0) T.parent[rc] = i; else T.leaf_parent[dec_leaf(rc)] = i;
+    if (i == 0) T.parent[0] = -1;
+}
+
+// Octree cells: kept[i] = 1 when node i starts a deeper cell than its parent.
+__global__ __launch_bounds__(256) void kept_kernel(const int* __restrict__ level, const int* __restrict__ parent,
+                                                   int m, int* __restrict__ kept, int* __restrict__ child) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    kept[i] = (i == 0 || level[i] > level[parent[i]]) ? 1 : 0;
+    for (int c = 0; c < 8; ++c) child[8 * i + c] = kEmpty;
+}
+
+// Every kept node (but the root) and every particle outside a bucket registers
+// in the slot of its octant in the nearest kept ancestor.
+__global__ __launch_bounds__(256) void link_kernel(const unsigned long long* __restrict__ keys, int n,
+                                                   const int* __restrict__ parent,
+                                                   const int* __restrict__ leaf_parent,
+                                                   const int* __restrict__ first, const int* __restrict__ level,
+                                                   const int* __restrict__ kept, int* __restrict__ child) {
+    const int e = blockIdx.x * 256 + threadIdx.x;  // [0, n-1): internal nodes, [n-1, 2n-1): leaves
+    const int m = n - 1;
+    if (e >= m + n) return;
+    int q, code;
+    unsigned long long key;
+    if (e < m) {
+        if (e == 0 || !kept[e]) return;
+        q = parent[e];
+        code = e;
+        key = keys[first[e]];
+    } else {
+        const int k = e - m;
+        q = leaf_parent[k];
+        code = enc_leaf(k);
+        key = keys[k];
+    }
+    while (!kept[q]) q = parent[q];
+    const int L = level[q];
+    if (L >= kBucketLevel) return;  // particles of a bucket are reached by its range
+    const int slot = static_cast<int>((key >> (60 - 3 * L)) & 7ULL);
+    child[8 * q + slot] = code;
+}
+
+// ---- 5. FP64 mass and centre of mass ------------------------------------------------
+
+struct MassArgs {
+    const float4* pos_sorted;
+    const unsigned* order;   // sorted position -> particle
+    const double* mass;      // by particle
+    const int* left;
+    const int* right;
+    const int* parent;
+    const int* leaf_parent;
+    unsigned* visit;         // zeroed
+    double4* sums;           // [n-1] (M, Mx, My, Mz)
+    int n;
+};
+
+__device__ inline double4 leaf_sum(const MassArgs& A, int k) {
+    const double m = A.mass[A.order[k]];
+    const float4 p = A.pos_sorted[k];
+    return make_double4(m, m * static_cast<double>(p.x), m * static_cast<double>(p.y), m * static_cast<double>(p.z));
+}
+
+__global__ __launch_bounds__(256) void mass_kernel(MassArgs A) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= A.n) return;
+    int node = A.leaf_parent[k];
+    while (node >= 0) {
+        __threadfence();
+        const unsigned prev = atomicAdd(&A.visit[node], 1u);
+        if (prev == 0) return;  // the sibling subtree finishes this node
+        __threadfence();
+        const int lc = A.left[node], rc = A.right[node];
+        double4 a, b;
+        if (lc < 0) {
+            a = leaf_sum(A, dec_leaf(lc));
+        } else {
+            volatile const double* s = &A.sums[lc].x;
+            a = make_double4(s[0], s[1], s[2], s[3]);
+        }
+        if (rc < 0) {
+            b = leaf_sum(A, dec_leaf(rc));
+        } else {
+            volatile const double* s = &A.sums[rc].x;
+            b = make_double4(s[0], s[1], s[2], s[3]);
+        }
+        A.sums[node] = make_double4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+        node = A.parent[node];
+    }
+}
+
+// Traversal records of the kept cells: (COM, G*M) in FP32 and the cell side.
+__global__ __launch_bounds__(256) void finalize_kernel(const double4* __restrict__ sums, const int* __restrict__ kept,
+                                                       const int* __restrict__ level, int m, double G,
+                                                       const RootParams* __restrict__ Rp,
+                                                       float4* __restrict__ com, float* __restrict__ side) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m || !kept[i]) return;
+    const double4 s = sums[i];
+    float cx = 0.f, cy = 0.f, cz = 0.f;
+    if (s.x > 0.0) {
+        cx = static_cast<float>(s.y / s.x);
+        cy = static_cast<float>(s.z / s.x);
+        cz = static_cast<float>(s.w / s.x);
+    }
+    com[i] = make_float4(cx, cy, cz, static_cast<float>(G * s.x));
+    side[i] = ldexpf(Rp->size, -level[i]);
+}
+
+__global__ __launch_bounds__(256) void gather_kernel(const float4* __restrict__ pos, const unsigned* __restrict__ order,
+                                                     int n, float4* __restrict__ out) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) out[k] = pos[order[k]];
+}
+
+// ---- 6. traversal ---------------------------------------------------------------------
+
+struct TravArgs {
+    const float4* pos_sorted;
+    const unsigned* order;
+    const float4* node_com;
+    const float* node_side;
+    const int* node_level;
+    const int* node_child;
+    const int* node_first;
+    const int* node_last;
+    const float4* pos_in;   // by particle
+    float4* pos_out;        // kick: by particle
+    float4* vel;
+    float4* acc;
+    unsigned long long* interactions;
+    unsigned* overflow;
+    int n;
+    int kick;
+    float theta2;    // theta^2 (pre-test)
+    float theta_lo;  // largest float below theta: approximate iff side / r <= theta_lo
+    float soft;
+    float tiny;
+    float dt;
+    float dts;
+};
+
+template <int LAW>
+__device__ __forceinline__ void pair_acc(float sx, float sy, float sz, float gm, float px, float py, float pz,
+                                         float soft, float tiny, float& ax, float& ay, float& az) {
+    const float dx = sx - px, dy = sy - py, dz = sz - pz;
+    float s;
+    if constexpr (LAW == NBODY_LAW_R) {
+        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        const float t = r2 + soft;
+        s = gm * __builtin_amdgcn_rsqf(__builtin_fmaf(r2 * t, t, tiny));
+    } else {
+        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, soft)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        s = (gm * inv) * (inv * inv);
+    }
+    ax = __builtin_fmaf(dx, s, ax);
+    ay = __builtin_fmaf(dy, s, ay);
+    az = __builtin_fmaf(dz, s, az);
+}
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// The reference's opening test, Octree.cpp:122-124: approximate iff
+// side / |p - COM| < theta, with |p - COM| = sqrtf(dx*dx + dy*dy + dz*dz).  A
+// squared comparison settles all but a thin band around the threshold, where
+// the correctly rounded FP32 quotient is formed as in the reference.
+__device__ __forceinline__ bool approximate(float side, float dx, float dy, float dz, float theta2, float theta_lo) {
+    const float r2 = (dx * dx + dy * dy) + dz * dz;
+    const float lhs = side * side;
+    const float rhs = theta2 * r2;
+    if (lhs < rhs * 0.999998f) return true;
+    if (lhs > rhs * 1.000002f) return false;
+    const float r = __fsqrt_rn(r2);
+    const float q = __fdiv_rn(side, r);
+    return q <= theta_lo;
+}
+
+template <int LAW>
+__global__ __launch_bounds__(kTravThreads) void traverse_kernel(TravArgs A) {
+    __shared__ int stk_node[kTravWaves][kStackCap];
+    __shared__ unsigned long long stk_mask[kTravWaves][kStackCap];
+    __shared__ unsigned long long red[kTravThreads];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int t = blockIdx.x * kTravThreads + tid;
+    const bool valid = t < A.n;
+    const int me_k = valid ? t : A.n - 1;
+    const float4 me = A.pos_sorted[me_k];
+    float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+    unsigned long long count = 0;
+
+    int sp = 0;
+    const unsigned long long all = __ballot(valid);
+    if (all) {
+        if (lane == 0) {
+            stk_node[wave][0] = 0;
+            stk_mask[wave][0] = all;
+        }
+        sp = 1;
+    }
+    while (sp > 0) {
+        --sp;
+        __builtin_amdgcn_wave_barrier();
+        const int node = __builtin_amdgcn_readfirstlane(stk_node[wave][sp]);
+        const unsigned long long mask_v = stk_mask[wave][sp];
+        const unsigned lo32 = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(mask_v));
+        const unsigned hi32 = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(mask_v >> 32));
+        const unsigned long long mask = (static_cast<unsigned long long>(hi32) << 32) | lo32;
+        const bool mine = (mask >> lane) & 1ULL;
+        float bx = 0.f, by = 0.f, bz = 0.f;
+        const int level = A.node_level[node];
+        if (level >= kBucketLevel) {
+            // identical keys: every particle of the range exactly, skipping self
+            const int f = A.node_first[node], l = A.node_last[node];
+            for (int q = f; q <= l; ++q) {
+                const float4 s = A.pos_sorted[q];
+                if (mine && q != t) {
+                    pair_acc<LAW>(s.x, s.y, s.z, s.w, me.x, me.y, me.z, A.soft, A.tiny, bx, by, bz);
+                    ++count;
+                }
+            }
+        } else {
+            const float4 c = A.node_com[node];
+            bool open = false;
+            if (mine) {
+                if (approximate(A.node_side[node], me.x - c.x, me.y - c.y, me.z - c.z, A.theta2, A.theta_lo)) {
+                    pair_acc<LAW>(c.x, c.y, c.z, c.w, me.x, me.y, me.z, A.soft, A.tiny, bx, by, bz);
+                    ++count;
+                } else {
+                    open = true;
+                }
+            }
+            const unsigned long long om = __ballot(open);
+            if (om) {
+                for (int slot = 0; slot < 8; ++slot) {
+                    const int ch = A.node_child[8 * node + slot];
+                    if (ch == kEmpty) continue;
+                    if (ch <= -2) {
+                        const int q = dec_leaf(ch);
+                        const float4 s = A.pos_sorted[q];
+                        if (open && q != t) {
+                            pair_acc<LAW>(s.x, s.y, s.z, s.w, me.x, me.y, me.z, A.soft, A.tiny, bx, by, bz);
+                            ++count;
+                        }
+                    } else {
+                        if (sp < kStackCap) {
+                            if (lane == 0) {
+                                stk_node[wave][sp] = ch;
+                                stk_mask[wave][sp] = om;
+                            }
+                            ++sp;
+                        } else if (lane == 0) {
+                            atomicOr(A.overflow, 1u);
+                        }
+                    }
+                }
+            }
+        }
+        kahan_add(ax, cx, bx);
+        kahan_add(ay, cy, by);
+        kahan_add(az, cz, bz);
+        __builtin_amdgcn_wave_barrier();
+    }
+
+    if (valid) {
+        const unsigned p = A.order[t];
+        A.acc[p] = make_float4(ax, ay, az, 0.f);
+        if (A.kick) {
+            float4 v = A.vel[p];
+            float4 x = A.pos_in[p];
+            v.x = __builtin_fmaf(ax, A.dt, v.x);
+            v.y = __builtin_fmaf(ay, A.dt, v.y);
+            v.z = __builtin_fmaf(az, A.dt, v.z);
+            x.x = __builtin_fmaf(v.x, A.dts, x.x);
+            x.y = __builtin_fmaf(v.y, A.dts, x.y);
+            x.z = __builtin_fmaf(v.z, A.dts, x.z);
+            A.vel[p] = v;
+            A.pos_out[p] = x;
+        }
+    }
+    red[tid] = count;
+    __syncthreads();
+    for (int w = kTravThreads / 2; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    if (tid == 0) atomicAdd(A.interactions, red[0]);
+}
+
+}  // namespace
+
+// ---- host side ----------------------------------------------------------------------
+
+struct BHState {
+    size_t cap = 0;  // particles the buffers hold
+    RootParams* root = nullptr;
+    float* bounds_partial = nullptr;
+    unsigned long long* keys[2] = {nullptr, nullptr};
+    unsigned* idx[2] = {nullptr, nullptr};
+    unsigned* hist = nullptr;
+    size_t hist_elems = 0;
+    float4* pos_sorted = nullptr;
+    int* left = nullptr;
+    int* right = nullptr;
+    int* parent = nullptr;
+    int* leaf_parent = nullptr;
+    int* first = nullptr;
+    int* last = nullptr;
+    int* level = nullptr;
+    int* kept = nullptr;
+    int* child = nullptr;
+    unsigned* visit = nullptr;
+    double4* sums = nullptr;
+    float4* com = nullptr;
+    float* side = nullptr;
+    unsigned* counters = nullptr;            // [0] outside, [1] overflow
+    unsigned long long* interactions = nullptr;
+    int sorted_buf = 0;                      // which keys/idx buffer holds the sorted result
+    RootParams fixed{};
+    bool fixed_root = false;
+};
+
+namespace {
+
+constexpr int kBoundsBlocks = 512;
+
+template <class T>
+void bh_free(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void bh_release(BHState* b) {
+    bh_free(b->bounds_partial);
+    bh_free(b->keys[0]);
+    bh_free(b->keys[1]);
+    bh_free(b->idx[0]);
+    bh_free(b->idx[1]);
+    bh_free(b->hist);
+    bh_free(b->pos_sorted);
+    bh_free(b->left);
+    bh_free(b->right);
+    bh_free(b->parent);
+    bh_free(b->leaf_parent);
+    bh_free(b->first);
+    bh_free(b->last);
+    bh_free(b->level);
+    bh_free(b->kept);
+    bh_free(b->child);
+    bh_free(b->visit);
+    bh_free(b->sums);
+    bh_free(b->com);
+    bh_free(b->side);
+    b->cap = 0;
+    b->hist_elems = 0;
+}
+
+template <class T>
+nbody_status bh_alloc(T*& p, size_t count) {
+    bh_free(p);
+    NB_HIP(hipMalloc(reinterpret_cast<void**>(&p), (count ? count : 1) * sizeof(T)));
+    return NBODY_OK;
+}
+
+inline int blocks_for(size_t n, int per) { return static_cast<int>((n + per - 1) / per); }
+
+float theta_below(double theta) {
+    float t = static_cast<float>(theta);
+    if (static_cast<double>(t) >= theta) t = std::nextafter(t, -INFINITY);
+    return t;
+}
+
+}  // namespace
+
+// The fixed root's exact quantisation parameters; fails when the corner is not
+// a multiple of the level-21 cell.
+nbody_status bh_fixed_root(const nbody_config* c, void* out_params) {
+    RootParams R{};
+    int e;
+    const float m = std::frexp(c->root_size, &e);
+    long long mi = static_cast<long long>(std::ldexp(m, 24));
+    int p = e - 24;
+    while ((mi & 1) == 0) {
+        mi >>= 1;
+        ++p;
+    }
+    NB_CHECK(mi < (1LL << 24) && 21 - p > -100 && 21 - p < 100, NBODY_ERR_INVALID_ARG, "unusable root size");
+    R.size = c->root_size;
+    R.odd = static_cast<int>(mi);
+    R.scale2 = std::ldexp(1.0f, 21 - p);
+    for (int a = 0; a < 3; ++a) {
+        R.lo[a] = c->root_lo[a];
+        const double L = std::floor(static_cast<double>(c->root_lo[a]) * static_cast<double>(R.scale2));
+        NB_CHECK(static_cast<double>(c->root_lo[a]) * static_cast<double>(R.scale2) == L && std::fabs(L) < 9e15,
+                 NBODY_ERR_INVALID_ARG, "fixed root corner must lie on the level-21 grid of the root");
+        const long long Li = static_cast<long long>(L);
+        NB_CHECK(Li % R.odd == 0, NBODY_ERR_INVALID_ARG,
+                 "fixed root corner must be a multiple of root_size / 2^21");
+        R.base[a] = floor_div_ll(Li, R.odd);
+    }
+    if (out_params) *static_cast<RootParams*>(out_params) = R;
+    return NBODY_OK;
+}
+
+nbody_status bh_create(nbody_sim* s) {
+    BHState* b = new (std::nothrow) BHState();
+    NB_CHECK(b, NBODY_ERR_OUT_OF_MEMORY, "host allocation failed");
+    if (s->cfg.root_mode == NBODY_ROOT_FIXED) {
+        nbody_status st = bh_fixed_root(&s->cfg, &b->fixed);
+        if (st != NBODY_OK) {
+            delete b;
+            return st;
+        }
+        b->fixed_root = true;
+    }
+    hipError_t e1 = hipMalloc(reinterpret_cast<void**>(&b->root), sizeof(RootParams));
+    hipError_t e2 = hipMalloc(reinterpret_cast<void**>(&b->counters), 2 * sizeof(unsigned));
+    hipError_t e3 = hipMalloc(reinterpret_cast<void**>(&b->interactions), sizeof(unsigned long long));
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        bh_destroy(b);
+        set_error("hipMalloc failed for Barnes-Hut control blocks");
+        return NBODY_ERR_OUT_OF_MEMORY;
+    }
+    if (b->fixed_root) {
+        hipError_t e = hipMemcpy(b->root, &b->fixed, sizeof(RootParams), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            bh_destroy(b);
+            set_error("hipMemcpy failed: %s", hipGetErrorString(e));
+            return NBODY_ERR_DEVICE;
+        }
+    }
+    (void)hipMemset(b->counters, 0, 2 * sizeof(unsigned));
+    (void)hipMemset(b->interactions, 0, sizeof(unsigned long long));
+    s->bh = b;
+    return NBODY_OK;
+}
+
+void bh_destroy(BHState* b) {
+    if (!b) return;
+    bh_release(b);
+    bh_free(b->root);
+    bh_free(b->counters);
+    bh_free(b->interactions);
+    delete b;
+}
+
+nbody_status bh_resize(BHState* b, size_t n, nbody_sim*) {
+    if (n <= b->cap && b->cap) return NBODY_OK;
+    bh_release(b);
+    const size_t m = n > 1 ? n - 1 : 1;
+    NB_TRY(bh_alloc(b->bounds_partial, 6 * kBoundsBlocks));
+    NB_TRY(bh_alloc(b->keys[0], n));
+    NB_TRY(bh_alloc(b->keys[1], n));
+    NB_TRY(bh_alloc(b->idx[0], n));
+    NB_TRY(bh_alloc(b->idx[1], n));
+    const size_t nb = static_cast<size_t>(blocks_for(n, kSortTile));
+    NB_TRY(bh_alloc(b->hist, 256 * nb));
+    b->hist_elems = 256 * nb;
+    NB_TRY(bh_alloc(b->pos_sorted, n));
+    NB_TRY(bh_alloc(b->left, m));
+    NB_TRY(bh_alloc(b->right, m));
+    NB_TRY(bh_alloc(b->parent, m));
+    NB_TRY(bh_alloc(b->leaf_parent, n));
+    NB_TRY(bh_alloc(b->first, m));
+    NB_TRY(bh_alloc(b->last, m));
+    NB_TRY(bh_alloc(b->level, m));
+    NB_TRY(bh_alloc(b->kept, m));
+    NB_TRY(bh_alloc(b->child, 8 * m));
+    NB_TRY(bh_alloc(b->visit, m));
+    NB_TRY(bh_alloc(b->sums, m));
+    NB_TRY(bh_alloc(b->com, m));
+    NB_TRY(bh_alloc(b->side, m));
+    b->cap = n;
+    return NBODY_OK;
+}
+
+// Keys, sort, tree, masses: everything before the traversal.
+static nbody_status bh_build(nbody_sim* s, BHState* b) {
+    const int n = static_cast<int>(s->n);
+    hipStream_t st = s->stream;
+    const float4* pos = s->pos[s->cur];
+    if (!b->fixed_root) {
+        hipLaunchKernelGGL(bounds_kernel, dim3(kBoundsBlocks), dim3(256), 0, st, pos, n, b->bounds_partial);
+        hipLaunchKernelGGL(dynamic_root_kernel, dim3(1), dim3(64), 0, st, b->bounds_partial, kBoundsBlocks, b->root);
+    }
+    hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, pos, n, b->root, b->keys[0],
+                       b->idx[0], b->counters);
+    const int nb = blocks_for(n, kSortTile);
+    int cur = 0;
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 8 * pass;
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], n, shift, b->hist, nb);
+        hipLaunchKernelGGL(radix_scan_kernel, dim3(1), dim3(kScanThreads), 0, st, b->hist, 256 * nb);
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], b->idx[cur],
+                           b->keys[cur ^ 1], b->idx[cur ^ 1], n, shift, b->hist, nb);
+        cur ^= 1;
+    }
+    b->sorted_buf = cur;
+    const unsigned long long* keys = b->keys[cur];
+    const unsigned* order = b->idx[cur];
+    hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, pos, order, n, b->pos_sorted);
+    const int m = n - 1;
+    TreeArgs T{keys, n, b->left, b->right, b->parent, b->leaf_parent, b->first, b->last, b->level};
+    hipLaunchKernelGGL(radix_tree_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, T);
+    hipLaunchKernelGGL(kept_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->level, b->parent, m, b->kept,
+                       b->child);
+    hipLaunchKernelGGL(link_kernel, dim3(blocks_for(m + n, 256)), dim3(256), 0, st, keys, n, b->parent,
+                       b->leaf_parent, b->first, b->level, b->kept, b->child);
+    NB_HIP(hipMemsetAsync(b->visit, 0, static_cast<size_t>(m) * sizeof(unsigned), st));
+    MassArgs M{b->pos_sorted, order, s->mass_dev, b->left, b->right, b->parent, b->leaf_parent, b->visit, b->sums, n};
+    hipLaunchKernelGGL(mass_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, M);
+    hipLaunchKernelGGL(finalize_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->sums, b->kept, b->level, m,
+                       s->cfg.G, b->root, b->com, b->side);
+    NB_HIP(hipGetLastError());
+    return NBODY_OK;
+}
+
+nbody_status bh_step(nbody_sim* s, float dt, bool kick, hipEvent_t ev_b, hipEvent_t ev_e) {
+    BHState* b = s->bh;
+    NB_CHECK(b, NBODY_ERR_NOT_INITIALISED, "Barnes-Hut state missing");
+    const int n = static_cast<int>(s->n);
+    hipStream_t st = s->stream;
+    if (n < 2) {
+        // A single particle feels no force; the direct pass gives exactly that.
+        DirectLaunch L{};
+        L.src = s->pos[s->cur];
+        L.pos_out = s->pos[s->cur ^ 1];
+        L.vel = s->vel;
+        L.acc = s->acc;
+        L.part = s->part;
+        L.n = n;
+        L.n_pad = static_cast<int>(s->n_pad);
+        L.law = s->cfg.law;
+        L.soft = static_cast<float>(s->cfg.softening);
+        L.tiny = direct_tiny_for(s->max_gm);
+        L.dt = dt;
+        L.dts = static_cast<float>(static_cast<double>(dt) / s->cfg.scale);
+        NB_HIP(direct_step(L, s->shape, kick, st, ev_b, ev_e));
+        return NBODY_OK;
+    }
+    hipEvent_t tb = nullptr, te = nullptr;
+    if (s->timing) {
+        NB_HIP(hipEventCreate(&tb));
+        s->tree_ms_pending.push_back(tb);
+        NB_HIP(hipEventCreate(&te));
+        s->tree_ms_pending.push_back(te);
+        NB_HIP(hipEventRecord(tb, st));
+    }
+    NB_TRY(bh_build(s, b));
+    if (te) NB_HIP(hipEventRecord(te, st));
+
+    TravArgs A{};
+    A.pos_sorted = b->pos_sorted;
+    A.order = b->idx[b->sorted_buf];
+    A.node_com = b->com;
+    A.node_side = b->side;
+    A.node_level = b->level;
+    A.node_child = b->child;
+    A.node_first = b->first;
+    A.node_last = b->last;
+    A.pos_in = s->pos[s->cur];
+    A.pos_out = s->pos[s->cur ^ 1];
+    A.vel = s->vel;
+    A.acc = s->acc;
+    A.interactions = b->interactions;
+    A.overflow = b->counters + 1;
+    A.n = n;
+    A.kick = kick ? 1 : 0;
+    A.theta2 = static_cast<float>(s->cfg.theta * s->cfg.theta);
+    A.theta_lo = theta_below(s->cfg.theta);
+    A.soft = static_cast<float>(s->cfg.softening);
+    A.tiny = direct_tiny_for(s->max_gm);
+    A.dt = dt;
+    A.dts = static_cast<float>(static_cast<double>(dt) / s->cfg.scale);
+    if (ev_b) NB_HIP(hipEventRecord(ev_b, st));
+    if (s->cfg.law == NBODY_LAW_R)
+        hipLaunchKernelGGL(traverse_kernel<NBODY_LAW_R>, dim3(blocks_for(n, kTravThreads)), dim3(kTravThreads), 0, st, A);
+    else
+        hipLaunchKernelGGL(traverse_kernel<NBODY_LAW_P>, dim3(blocks_for(n, kTravThreads)), dim3(kTravThreads), 0, st, A);
+    NB_HIP(hipGetLastError());
+    if (ev_e) NB_HIP(hipEventRecord(ev_e, st));
+    return NBODY_OK;
+}
+
+nbody_status bh_reset_counters(BHState* b, hipStream_t st) {
+    NB_HIP(hipMemsetAsync(b->counters, 0, 2 * sizeof(unsigned), st));
+    NB_HIP(hipMemsetAsync(b->interactions, 0, sizeof(unsigned long long), st));
+    return NBODY_OK;
+}
+
+nbody_status bh_read_counters(BHState* b, hipStream_t st, unsigned long long* interactions, unsigned* outside,
+                              unsigned* overflow) {
+    unsigned c[2] = {0, 0};
+    unsigned long long it = 0;
+    NB_HIP(hipMemcpyAsync(c, b->counters, sizeof c, hipMemcpyDeviceToHost, st));
+    NB_HIP(hipMemcpyAsync(&it, b->interactions, sizeof it, hipMemcpyDeviceToHost, st));
+    NB_HIP(hipStreamSynchronize(st));
+    if (interactions) *interactions = it;
+    if (outside) *outside = c[0];
+    if (overflow) *overflow = c[1];
+    return NBODY_OK;
+}
+
+nbody_status bh_export_order(nbody_sim* s, uint64_t* keys, uint32_t* order, size_t n) {
+    BHState* b = s->bh;
+    NB_CHECK(b && b->cap >= n && n == s->n, NBODY_ERR_NOT_INITIALISED, "no Barnes-Hut tree of %zu particles", n);
+    if (n == 0) return NBODY_OK;
+    if (keys)
+        NB_HIP(hipMemcpyAsync(keys, b->keys[b->sorted_buf], n * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+    if (order)
+        NB_HIP(hipMemcpyAsync(order, b->idx[b->sorted_buf], n * sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream));
+    NB_HIP(hipStreamSynchronize(s->stream));
+    return NBODY_OK;
+}
+
 }  // namespace nbody

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cstdint>
 
 #include "../../include/nbody.h"
 
@@ -20,5 +21,13 @@ void bh_destroy(BHState* b);
 nbody_status bh_resize(BHState* b, size_t n, nbody_sim* s);
 // One force evaluation (kick: a full step) of the handle's current state.
 nbody_status bh_step(nbody_sim* s, float dt, bool kick, hipEvent_t ev_force_begin, hipEvent_t ev_force_end);
+// Check a fixed root (corner on the root's level-21 grid); out_params may be null.
+nbody_status bh_fixed_root(const nbody_config* c, void* out_params);
+// Interaction / clamp / overflow counters of the steps since the last reset.
+nbody_status bh_reset_counters(BHState* b, hipStream_t st);
+nbody_status bh_read_counters(BHState* b, hipStream_t st, unsigned long long* interactions, unsigned* outside,
+                              unsigned* overflow);
+// Sorted keys and particle order of the last tree build.
+nbody_status bh_export_order(nbody_sim* s, uint64_t* keys, uint32_t* order, size_t n);
 
 }  // namespace nbody

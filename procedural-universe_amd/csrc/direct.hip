@@ -32,7 +32,7 @@
 // that single rate.  So every per-pair operation is a VGPR-only
 // add/sub/mul/fma (12 of them) plus one v_rsq_f32: the softening and `tiny`
 // are copied into VGPRs once per workgroup, and the coincident-pair guard is
-// an FMA rather than a clamp.
+// an FMA rather than a clamp.  The rsq of four pairs are issued together.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -59,24 +59,60 @@ __device__ __forceinline__ float in_vgpr(float x) {
     return v;
 }
 
+// Four reciprocal square roots issued back to back.  Interleaved with the
+// dual-issued FP32 arithmetic a lone v_rsq_f32 stalls the SIMD for far longer
+// than its own issue slots; grouped, the transcendental unit stays busy while
+// the surrounding arithmetic of other waves pairs up.  The trailing s_nop
+// covers the transcendental-result forwarding wait before the first use.
+// The compiler itself separates a v_rsq_f32 result from its first VALU use by
+// one wait state; the group ends with NBODY_RSQ_NOP + 1 of them.
+#ifndef NBODY_RSQ_NOP
+#define NBODY_RSQ_NOP 1
+#endif
+// Source groups of four unrolled per inner-loop iteration.
+#ifndef NBODY_GROUP_UNROLL
+#define NBODY_GROUP_UNROLL 4
+#endif
+#define NBODY_STR2(x) #x
+#define NBODY_STR(x) NBODY_STR2(x)
+
+__device__ __forceinline__ void rsq4(float& a, float& b, float& c, float& d) {
+    asm volatile(
+        "v_rsq_f32 %0, %0\n\t"
+        "v_rsq_f32 %1, %1\n\t"
+        "v_rsq_f32 %2, %2\n\t"
+        "v_rsq_f32 %3, %3\n\t"
+        "s_nop " NBODY_STR(NBODY_RSQ_NOP)
+        : "+v"(a), "+v"(b), "+v"(c), "+v"(d));
+}
+
+// Argument of the reciprocal square root for one pair: r2 (r2 + S)^2 + tiny
+// (law R) or r2 + eps^2 (law P), with the difference vector d = x_j - x_i.
 template <int LAW>
-__device__ __forceinline__ void interact(const float4 pj, const float xi, const float yi,
-                                         const float zi, const float soft, const float tiny,
-                                         float& ax, float& ay, float& az) {
-    const float dx = pj.x - xi;
-    const float dy = pj.y - yi;
-    const float dz = pj.z - zi;
-    float s;
+__device__ __forceinline__ float pair_arg(const float4 pj, const float xi, const float yi, const float zi,
+                                          const float soft, const float tiny, float& dx, float& dy, float& dz) {
+    dx = pj.x - xi;
+    dy = pj.y - yi;
+    dz = pj.z - zi;
     if constexpr (LAW == NBODY_LAW_R) {
         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
         const float t = r2 + soft;
         const float u = r2 * t;
-        const float q = __builtin_fmaf(u, t, tiny);
-        s = pj.w * __builtin_amdgcn_rsqf(q);
+        return __builtin_fmaf(u, t, tiny);
     } else {
-        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, soft)));
-        const float inv = __builtin_amdgcn_rsqf(r2);
-        s = (pj.w * inv) * (inv * inv);
+        return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, soft)));
+    }
+}
+
+// acc += d * G m_j * f(rsq): law R uses rsq directly, law P its cube.
+template <int LAW>
+__device__ __forceinline__ void pair_acc(const float gm, const float inv, const float dx, const float dy,
+                                         const float dz, float& ax, float& ay, float& az) {
+    float s;
+    if constexpr (LAW == NBODY_LAW_R) {
+        s = gm * inv;
+    } else {
+        s = (gm * inv) * (inv * inv);
     }
     ax = __builtin_fmaf(dx, s, ax);
     ay = __builtin_fmaf(dy, s, ay);
@@ -135,11 +171,22 @@ __global__ __launch_bounds__(BLOCK) void direct_force_kernel(DirectLaunch L, int
 #pragma unroll
         for (int k = 0; k < K; ++k) ax[k] = ay[k] = az[k] = 0.f;
 
-#pragma unroll 4
-        for (int j = 0; j < kTile; ++j) {
-            const float4 pj = tile[j];
+        // Groups of four sources: four rsq arguments per target, then one
+        // grouped rsq4 per target, then the accumulation.
+#pragma unroll NBODY_GROUP_UNROLL
+        for (int j = 0; j < kTile; j += 4) {
+            float4 pj[4];
 #pragma unroll
-            for (int k = 0; k < K; ++k) interact<LAW>(pj, xi[k], yi[k], zi[k], soft, tiny, ax[k], ay[k], az[k]);
+            for (int g = 0; g < 4; ++g) pj[g] = tile[j + g];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                float dx[4], dy[4], dz[4], q[4];
+#pragma unroll
+                for (int g = 0; g < 4; ++g) q[g] = pair_arg<LAW>(pj[g], xi[k], yi[k], zi[k], soft, tiny, dx[g], dy[g], dz[g]);
+                rsq4(q[0], q[1], q[2], q[3]);
+#pragma unroll
+                for (int g = 0; g < 4; ++g) pair_acc<LAW>(pj[g].w, q[g], dx[g], dy[g], dz[g], ax[k], ay[k], az[k]);
+            }
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -254,16 +301,12 @@ int direct_effective_split(const DirectShape& s, int n_pad) {
 }
 
 DirectShape direct_default_shape(int n_pad) {
-    // Fill the 256 CUs with at least ~1024 workgroups.
-    DirectShape s{4, 256, 1};
-    int iblocks = (n_pad + s.block * s.i_per_lane - 1) / (s.block * s.i_per_lane);
-    if (iblocks < 1024) {
-        s.i_per_lane = 2;
-        s.block = 128;
-        iblocks = (n_pad + 255) / 256;
-        int split = (1024 + iblocks - 1) / iblocks;
-        s.split = split;
-    }
+    // One target per lane keeps the register count, and so the waves per SIMD,
+    // highest; below ~1024 workgroups of targets the j-range is split so that
+    // the 256 CUs still see about a thousand workgroups.
+    DirectShape s{1, 256, 1};
+    const int iblocks = (n_pad + s.block - 1) / s.block;
+    if (iblocks < 1024) s.split = (1024 + iblocks - 1) / iblocks;
     return s;
 }
 

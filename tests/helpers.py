@@ -59,6 +59,9 @@ class Oracle:
         L.oracle_fp64_accel.argtypes = [ctypes.c_int, _SZ, _F32P, _F64P, _D, _D, _I64P, _SZ, ctypes.c_int, _F64P]
         L.oracle_fp64_energy.argtypes = [ctypes.c_int, _SZ, _F32P, _F32P, _F64P, _D, _D, _D, ctypes.c_int, _F64P]
         L.oracle_bh_forces.argtypes = [_SZ, _F32P, _F64P, _D, _D, _D, _F32P, ctypes.c_float, ctypes.c_int, _F64P, _I64P]
+        L.oracle_bh_forces_ex.argtypes = [
+            _SZ, _F32P, _F64P, _D, _D, _D, _F32P, ctypes.c_float, ctypes.c_int, ctypes.c_int, _F64P, _I64P
+        ]
         L.oracle_bh_step.argtypes = [
             _SZ, _F32P, _F64P, _F64P, _D, _D, _D, _D, _F32P, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int
         ]
@@ -127,14 +130,15 @@ class Oracle:
             "momentum": out[2:5].copy(), "angular_momentum": out[5:8].copy(), "mass": out[8],
         }
 
-    def bh_forces(self, pos, mass, theta, root_lo, root_size, G=REF_G, S=REF_S, workers=0):
+    def bh_forces(self, pos, mass, theta, root_lo, root_size, G=REF_G, S=REF_S, workers=0, com_fp64=False):
         pos, mass = _f32(pos), _f64(mass)
         lo = _f32(root_lo)
         out = np.empty((mass.size, 3))
         stats = np.zeros(4, np.int64)
-        self.L.oracle_bh_forces(
+        self.L.oracle_bh_forces_ex(
             mass.size, pos.ctypes.data_as(_F32P), mass.ctypes.data_as(_F64P), G, S, theta,
-            lo.ctypes.data_as(_F32P), float(root_size), workers, out.ctypes.data_as(_F64P), stats.ctypes.data_as(_I64P)
+            lo.ctypes.data_as(_F32P), float(root_size), 1 if com_fp64 else 0, workers, out.ctypes.data_as(_F64P),
+            stats.ctypes.data_as(_I64P)
         )
         return out, {"nodes": int(stats[0]), "leaves": int(stats[1]), "max_depth": int(stats[2]), "dropped": int(stats[3])}
 

@@ -1,0 +1,202 @@
+"""Parity of the device Barnes-Hut with the CPU restatement of the reference's
+Octree / BarnesHut (src/Sim/Octree.cpp:16-145, src/Sim/BarnesHut.cpp:44-110).
+
+* Morton keys and the sorted order: bit-exact against the CPU.
+* Accelerations: the reference's opening test (side / |p - COM| < theta) and
+  monopoles on the same cells; per-particle relative error 1e-5 (the cell
+  centres of mass are FP64 here and FP32 in the reference).
+* Positions after 10 steps: relative error 1e-4.
+"""
+import numpy as np
+import pytest
+
+from helpers import REF_G, REF_S, rel_err_rows
+
+pytestmark = pytest.mark.gpu
+
+REF_LO = [-4000.0] * 3
+REF_SIZE = 8000.0
+
+
+def _bh(pu, cfg, pos, vel, mass):
+    sim = pu.NBodySim(cfg)
+    sim.init_soa(pos, vel, mass)
+    return sim
+
+
+def _ref_input(pu, n, seed, half=500.0):
+    return pu.ic_uniform(n, seed, half=half, mass_lo=1e20, mass_hi=1e30)
+
+
+def test_keys_and_order_bit_exact_fixed_root(gpu, oracle):
+    pu = gpu
+    pos, vel, mass = _ref_input(pu, 20000, 0x5EED0004)
+    pos[100:140] = pos[7]  # identical keys: the order among them must be stable
+    pos[500] = np.float32(0.0)
+    pos[501] = np.nextafter(np.float32(0), np.float32(-1))
+    with _bh(pu, pu.reference_config(pu.METHOD_BARNES_HUT), pos, vel, mass) as sim:
+        sim.accelerations()
+        keys, order = sim.bh_tree_order()
+        st = sim.bh_stats()
+    ref_keys, outside = oracle.morton_keys(pos, REF_LO, REF_SIZE)
+    ref_order = oracle.sort_order(ref_keys)
+    assert outside == 0 and st["outside_root"] == 0
+    assert np.array_equal(order, ref_order)
+    assert np.array_equal(keys, ref_keys[ref_order])
+
+
+def test_keys_and_order_bit_exact_dynamic_root(gpu, oracle):
+    pu = gpu
+    pos, vel, mass = pu.ic_plummer(50000, 0x5EED0005)
+    pos += np.float32(3.25)
+    cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=1e-4, theta=0.5)
+    with _bh(pu, cfg, pos, vel, mass) as sim:
+        sim.accelerations()
+        keys, order = sim.bh_tree_order()
+    lo, size = oracle.dynamic_root(pos)
+    ref_keys, outside = oracle.morton_keys(pos, lo, size)
+    ref_order = oracle.sort_order(ref_keys)
+    assert outside == 0
+    assert np.array_equal(order, ref_order)
+    assert np.array_equal(keys, ref_keys[ref_order])
+
+
+@pytest.mark.parametrize("theta", [2.0, 0.5, 1.0])
+def test_reference_octree_forces(gpu, oracle, theta):
+    """Reference constants, fixed root [-4000, 4000)^3 (BarnesHut.cpp:14-19)."""
+    pu = gpu
+    n = 4096
+    pos, vel, mass = _ref_input(pu, n, 0x5EED0006)
+    cfg = pu.reference_config(pu.METHOD_BARNES_HUT, theta=theta)
+    with _bh(pu, cfg, pos, vel, mass) as sim:
+        a = sim.accelerations()
+        st = sim.bh_stats()
+    F, info = oracle.bh_forces(pos, mass, theta, REF_LO, REF_SIZE)
+    ref = F / mass[:, None]
+    err = rel_err_rows(a, ref)
+    print(f"theta {theta}: max rel {err.max():.2e}, p99.9 {np.quantile(err, 0.999):.2e}, "
+          f"interactions/particle {st['interactions'] / n:.0f}, stack overflow {st['stack_overflow']}")
+    assert st["stack_overflow"] == 0
+    assert info["dropped"] == 0
+    assert err.max() < 1e-5
+
+
+def test_dynamic_root_forces(gpu, oracle):
+    """G = 1 Plummer sphere with the per-step power-of-two root; the CPU octree
+    is built over the same root."""
+    pu = gpu
+    n, soft, theta = 6000, 1e-4, 0.5
+    pos, vel, mass = pu.ic_plummer(n, 0x5EED0007)
+    cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=soft, theta=theta)
+    with _bh(pu, cfg, pos, vel, mass) as sim:
+        a = sim.accelerations()
+    lo, size = oracle.dynamic_root(pos)
+    F, _ = oracle.bh_forces(pos, mass, theta, lo, size, G=1.0, S=soft)
+    err = rel_err_rows(a, F / mass[:, None])
+    print(f"dynamic root: max rel {err.max():.2e}")
+    assert err.max() < 1e-5
+
+
+def test_theta_zero_is_direct_sum(gpu):
+    pu = gpu
+    n, soft = 3000, 1e-4
+    pos, vel, mass = pu.ic_plummer(n, 0x5EED0008)
+    cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=soft, theta=0.0)
+    with _bh(pu, cfg, pos, vel, mass) as sim:
+        a_bh = sim.accelerations()
+        st = sim.bh_stats()
+    with _bh(pu, pu.default_config(law=pu.LAW_R, softening=soft), pos, vel, mass) as sim:
+        a_d = sim.accelerations()
+    assert st["interactions"] == n * (n - 1)
+    assert rel_err_rows(a_bh, a_d).max() < 2e-6
+
+
+def test_reference_ten_steps(gpu, oracle):
+    pu = gpu
+    n = 2048
+    pos, _, mass = _ref_input(pu, n, 0x5EED0009)
+    vel = (pos / np.linalg.norm(pos, axis=1, keepdims=True) * 1e16).astype(np.float32)
+    dt = np.float32(0.02 / 60)
+    with _bh(pu, pu.reference_config(pu.METHOD_BARNES_HUT), pos, vel, mass) as sim:
+        sim.step(dt, 10)
+        p, v, _ = sim.read_soa()
+    p_ref, v_ref = oracle.bh_step(pos, vel.astype(np.float64), mass, dt, 10, 2.0, REF_LO, REF_SIZE)
+    perr = rel_err_rows(p, p_ref)
+    print(f"BH 10 steps: pos max rel {perr.max():.2e}")
+    assert perr.max() < 1e-4
+
+
+def test_identical_positions_form_a_bucket(gpu):
+    """Coincident particles share one 63-bit key; their cell is evaluated pair by pair."""
+    pu = gpu
+    pos, vel, mass = pu.ic_plummer(1000, 0x5EED000A)
+    pos[200:260] = pos[3]
+    soft = 1e-4
+    cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=soft, theta=0.0)
+    with _bh(pu, cfg, pos, vel, mass) as sim:
+        a_bh = sim.accelerations()
+    with _bh(pu, pu.default_config(law=pu.LAW_R, softening=soft), pos, vel, mass) as sim:
+        a_d = sim.accelerations()
+    assert np.all(np.isfinite(a_bh))
+    assert rel_err_rows(a_bh, a_d).max() < 2e-6
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17])
+def test_tiny_sizes(gpu, oracle, n):
+    pu = gpu
+    pos, vel, mass = _ref_input(pu, n, 300 + n, half=100.0)
+    with _bh(pu, pu.reference_config(pu.METHOD_BARNES_HUT), pos, vel, mass) as sim:
+        a = sim.accelerations()
+        sim.step(np.float32(1e-3), 2)
+        p, _, _ = sim.read_soa()
+    F, _ = oracle.bh_forces(pos, mass, 2.0, REF_LO, REF_SIZE)
+    if n == 1:
+        assert np.all(a == 0)
+    else:
+        assert rel_err_rows(a, F / mass[:, None]).max() < 1e-5
+    p_ref, _ = oracle.bh_step(pos, vel.astype(np.float64), mass, np.float32(1e-3), 2, 2.0, REF_LO, REF_SIZE)
+    assert rel_err_rows(p, p_ref).max() < 1e-4
+
+
+def test_outside_fixed_root_is_counted(gpu):
+    pu = gpu
+    pos, vel, mass = _ref_input(pu, 100, 5)
+    pos[0] = [5000.0, 0.0, 0.0]
+    with _bh(pu, pu.reference_config(pu.METHOD_BARNES_HUT), pos, vel, mass) as sim:
+        sim.accelerations()
+        assert sim.bh_stats()["outside_root"] == 1
+
+
+def test_two_disc_accuracy_vs_direct(gpu, oracle):
+    """Two-disc galaxies, theta = 0.5: parity with the CPU octree on the same
+    root, and the monopole approximation's own error against the device direct
+    sum (about 1% RMS for the reference's opening test at theta = 0.5)."""
+    pu = gpu
+    n, soft = 65536, 0.0025
+    p1, v1, m1 = pu.ic_disc_bulge(n // 2, 11, bulge_fraction=0.0)
+    p2, v2, m2 = pu.ic_disc_bulge(n // 2, 12, bulge_fraction=0.0)
+    pos = np.vstack([p1 - np.float32([20, 0, 0]), p2 + np.float32([20, 0, 0])]).astype(np.float32)
+    vel = np.vstack([v1, v2]).astype(np.float32)
+    mass = np.concatenate([m1, m2])
+    cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=soft, theta=0.5)
+    with _bh(pu, cfg, pos, vel, mass) as sim:
+        a_bh = sim.accelerations()
+        st = sim.bh_stats()
+    with _bh(pu, pu.default_config(law=pu.LAW_R, softening=soft), pos, vel, mass) as sim:
+        a_d = sim.accelerations()
+    lo, size = oracle.dynamic_root(pos)
+    # same cells and opening test with FP64 centres of mass: the traversal itself
+    F64, _ = oracle.bh_forces(pos, mass, 0.5, lo, size, G=1.0, S=soft, com_fp64=True)
+    perr = rel_err_rows(a_bh, F64 / mass[:, None])
+    # the reference's FP32 centre-of-mass accumulation differs by its own rounding
+    F32, _ = oracle.bh_forces(pos, mass, 0.5, lo, size, G=1.0, S=soft)
+    rerr = rel_err_rows(a_bh, F32 / mass[:, None])
+    err = rel_err_rows(a_bh, a_d)
+    rms = np.sqrt(np.mean(err ** 2))
+    print(f"two discs N={n}: vs CPU octree (FP64 COM) max {perr.max():.2e}; vs reference FP32 COM "
+          f"rms {np.sqrt(np.mean(rerr ** 2)):.2e} max {rerr.max():.2e}; vs direct rms {rms:.2e} max {err.max():.2e}; "
+          f"interactions/particle {st['interactions'] / n:.0f}")
+    assert perr.max() < 1e-5
+    assert np.sqrt(np.mean(rerr ** 2)) < 1e-4
+    assert rms < 3e-2
+    assert st["interactions"] < n * n / 4
