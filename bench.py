@@ -71,6 +71,195 @@ def cpu_model():
     return "unknown"
 
 
+def bench_batch(pu, args):
+    """512 independent galaxies (log-uniform 1,024..65,536 particles, Plummer or
+    exponential disc) advanced together by the persistent batch kernel in one
+    hipGraph launch, against each galaxy alone with plain launches."""
+    count, steps = args.galaxies, args.steps
+    t0 = time.time()
+    gal = pu.batch_galaxies(count, args.seed, 1024, 65536)
+    t_ic = time.time() - t0
+    total = sum(len(m) for (_, _, m) in gal)
+    cap = sum((len(m) + 255) // 256 * 256 for (_, _, m) in gal)
+    soft = args.eps * args.eps
+    cfg = pu.default_config(law=pu.LAW_R if args.law == "R" else pu.LAW_P, softening=soft)
+    b = pu.Batch(cfg, cap)
+    for p, v, m in gal:
+        b.add(p, v, m)
+    inf = b.info()
+    per_step = float(inf.interactions_per_step)
+    tuning = b.tune(args.dt)
+    if args.warmup:
+        b.step(args.dt, args.warmup)
+    b.synchronize()
+    t = time.perf_counter()
+    b.step(args.dt, steps)
+    b.synchronize()
+    elapsed = time.perf_counter() - t
+    tm = b.timing()
+    value = per_step * steps / elapsed
+    # Baselines on a fresh copy: every galaxy alone, one launch per step.
+    seq_steps = args.baseline_steps
+    s = pu.Batch(cfg, cap)
+    for p, v, m in gal:
+        s.add(p, v, m)
+    seq = {}
+    for split_small in (False, True):
+        s.synchronize()
+        t = time.perf_counter()
+        s.step_sequential(args.dt, seq_steps, split_small=split_small)
+        s.synchronize()
+        seq[split_small] = per_step * seq_steps / (time.perf_counter() - t)
+    s.close()
+    b.close()
+    seq_value = seq[False]
+    return {
+        "metric": "batched aggregate interactions/s (512 galaxies, one persistent hipGraph launch)",
+        "value": value,
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{count} galaxies, N log-uniform in [1024, 65536] (Plummer or exponential disc, 50/50), "
+            f"direct-sum FP32, eps={args.eps}, dt={args.dt}",
+            "galaxies": count,
+            "particles": total,
+            "interactions_per_step": per_step,
+            "tuning": tuning,
+            "graph_device_ms": tm.step_ms,
+            "ic_seconds": t_ic,
+        },
+        "baseline": {
+            "kind": "sequential per-galaxy plain launches (same kernel and arena, one workgroup per 256 targets, "
+            "synchronised per galaxy)",
+            "steps": seq_steps,
+            "value": seq_value,
+            "unit": "interactions/s",
+            "speedup": value / seq_value,
+            "tuned_single_system": {
+                "kind": "sequential per-galaxy launches of the single-system default (sources split over more "
+                "workgroups for small systems, plus a reduction launch)",
+                "value": seq[True],
+                "speedup": value / seq[True],
+            },
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": FLOP_PER_INTERACTION * value / 1e12,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": FLOP_PER_INTERACTION * value / 1e12 / FP32_PEAK_TFLOPS,
+            "traffic": None,
+        },
+    }
+
+
+def bench_bh(pu, args):
+    """Barnes-Hut, theta = 0.5, monopoles: two exponential-disc galaxies of
+    N/2 particles on a parabolic encounter (separation 40, pericentre 5, second
+    disc tilted 30 degrees)."""
+    n = args.n
+    soft = args.eps * args.eps
+    t0 = time.time()
+    pos, vel, mass = pu.ic_disc_encounter(n, args.seed, 3.0, 0.3, 40.0, 5.0, 30.0)
+    t_ic = time.time() - t0
+    cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=soft, theta=args.theta)
+    sim = pu.NBodySim(cfg)
+    sim.init_soa(pos, vel, mass)
+    sim.set_timing(True)
+    acc0 = sim.accelerations()
+    per_eval = sim.bh_stats()["interactions"]
+    keys, order = sim.bh_tree_order()
+    rng = np.random.default_rng(2024)
+    rows = np.sort(rng.choice(n, size=min(args.subsample, n), replace=False))
+    a_dir = sim.field(pos[rows])
+    err = np.linalg.norm(acc0[rows].astype(np.float64) - a_dir, axis=1) / np.linalg.norm(a_dir, axis=1)
+    if args.warmup:
+        sim.step(args.dt, args.warmup)
+    sim.synchronize()
+    t = time.perf_counter()
+    sim.step(args.dt, args.steps)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t
+    tm = sim.timing()
+    stats = sim.bh_stats()
+    interactions = float(stats["interactions"])
+    value = interactions / elapsed
+    force_ms = tm.force_ms / max(1, tm.force_launches)
+    tree_ms = tm.tree_ms / max(1, tm.force_launches)
+    result = {
+        "metric": "Barnes-Hut interactions/s (counted cell + particle interactions), theta=0.5, N=4,194,304",
+        "value": value,
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"two exponential discs of {n // 2} particles each, parabolic encounter (separation 40, "
+            f"pericentre 5, tilt 30 deg), Barnes-Hut theta={args.theta}, monopoles, 63-bit Morton keys, "
+            f"FP64 cell mass/COM, eps={args.eps}, dt={args.dt}",
+            "n": n,
+            "theta": args.theta,
+            "interactions_per_particle": interactions / (n * args.steps),
+            "direct_equivalent_interactions_per_s": float(n) * float(n) * args.steps / elapsed,
+            "tree_ms_per_step": tree_ms,
+            "traversal_ms_per_step": force_ms,
+            "stack_overflow": stats["stack_overflow"],
+            "ic_seconds": t_ic,
+        },
+        "accuracy_vs_direct": {
+            "reference": "device direct sum (FP32 + compensated sums) at the sampled particles, same state",
+            "rows": int(rows.size),
+            "rms_rel": float(np.sqrt(np.mean(err ** 2))),
+            "max_rel": float(err.max()),
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": FLOP_PER_INTERACTION * interactions / (args.steps * force_ms * 1e-3) / 1e12,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": FLOP_PER_INTERACTION * interactions / (args.steps * force_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "traverse_kernel",
+        },
+    }
+    if not args.no_cpu_baseline:
+        # CPU leg: the keys and the sorted order against the CPU computation on
+        # the full N, and one step of the restated reference octree timed.
+        oracle = load_oracle()
+        lo, size = oracle.dynamic_root(pos)
+        ref_keys, outside = oracle.morton_keys(pos, lo, size)
+        ref_order = oracle.sort_order(ref_keys)
+        cores = host_cores()
+        t = time.perf_counter()
+        oracle.bh_forces(pos, mass, args.theta, lo, size, G=1.0, S=soft, workers=cores)
+        ct = time.perf_counter() - t
+        result["keys_bit_exact"] = bool(np.array_equal(order, ref_order) and np.array_equal(keys, ref_keys[ref_order]))
+        result["cpu_baseline"] = {
+            "value": per_eval / ct,
+            "unit": "interactions/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": f"one full force evaluation of the restated reference octree (Octree::Add/CalculateMass/"
+            f"CalculateForce) on the same {n} particles ({ct:.1f} s, serial build), interactions counted on the device",
+        }
+    sim.close()
+    return result
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +274,11 @@ def main():
     ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--accuracy-rows", type=int, default=1024)
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU baseline sample (0 = auto)")
+    ap.add_argument("--workload", choices=["direct", "batch", "bh"], default="direct")
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--subsample", type=int, default=65536)
+    ap.add_argument("--galaxies", type=int, default=512)
+    ap.add_argument("--baseline-steps", type=int, default=10)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -100,6 +294,28 @@ def main():
     if pu.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(3)
+
+    if args.workload == "batch":
+        if "--eps" not in sys.argv:
+            args.eps = 0.01
+        if "--dt" not in sys.argv:
+            args.dt = 1e-3
+        if "--steps" not in sys.argv:
+            args.steps = 50
+        if "--seed" not in sys.argv:
+            args.seed = 0x5EED0004
+        print(json.dumps(bench_batch(pu, args)))
+        return
+
+    if args.workload == "bh":
+        if "--n" not in sys.argv:
+            args.n = 1 << 22
+        if "--steps" not in sys.argv:
+            args.steps = 10
+        if "--seed" not in sys.argv:
+            args.seed = 0x5EED0003
+        print(json.dumps(bench_bh(pu, args)))
+        return
 
     n = args.n
     law = pu.LAW_R if args.law == "R" else pu.LAW_P

@@ -173,6 +173,12 @@ nbody_status nbody_read_soa(nbody_sim* sim, float* pos, float* vel, float* acc, 
  * receives 3n floats.  Used for parity checks. */
 nbody_status nbody_compute_accelerations(nbody_sim* sim, float* acc, size_t n);
 
+/* Direct-sum acceleration at m arbitrary points (points[3m] -> acc[3m]) from
+ * the current particles, with the handle's law; a point that coincides with a
+ * particle gets no contribution from it.  Used to check Barnes-Hut forces on
+ * a subsample. */
+nbody_status nbody_field(nbody_sim* sim, const float* points, size_t m, float* acc);
+
 /* Number of particles of the last init. */
 nbody_status nbody_size(nbody_sim* sim, size_t* n);
 
@@ -230,6 +236,73 @@ nbody_status nbody_bh_stats_get(nbody_sim* sim, nbody_bh_stats* out);
 nbody_status nbody_bh_tree_order(nbody_sim* sim, uint64_t* keys, uint32_t* order, size_t n);
 
 /* ------------------------------------------------------------------------ */
+/* Batched direct sum: many independent systems in one device arena          */
+/* ------------------------------------------------------------------------ */
+
+/* The reference advances one particle set per INBodySim; a batch advances many
+ * independent systems (galaxies) with one persistent kernel per step that
+ * drains a queue of (system, target block) work items, largest first, over a
+ * shared device arena, and runs a captured hipGraph of the steps.  A system's
+ * forces are bit-identical whether it runs alone or in any mix.  Systems may
+ * be added and removed between steps (the server does so); a batch is not
+ * thread-safe. */
+typedef struct nbody_batch nbody_batch;
+
+typedef struct nbody_batch_tuning {
+    int32_t i_per_lane;   /* targets per lane: 1 or 2                              */
+    int32_t block;        /* threads per workgroup: 128 or 256                     */
+    int32_t wg_per_cu;    /* persistent workgroups launched per CU (1..64)         */
+    int32_t reserved[5];
+    double  ms_per_step;  /* the tuner's time for one step of this shape (0: none) */
+} nbody_batch_tuning;
+
+typedef struct nbody_batch_info {
+    int32_t  systems;
+    int32_t  work_items;
+    uint64_t particles;
+    uint64_t interactions_per_step;  /* sum of n^2 over the systems                 */
+    uint64_t capacity;               /* arena records                               */
+    uint64_t free_records;
+    uint64_t largest_free;           /* largest system (padded) that still fits      */
+    nbody_batch_tuning tuning;
+} nbody_batch_info;
+
+/* cfg->method must be NBODY_METHOD_DIRECT; capacity = arena size in particles
+ * (each system takes its size rounded up to 256). */
+nbody_status nbody_batch_create(const nbody_config* cfg, size_t capacity, nbody_batch** out);
+nbody_status nbody_batch_destroy(nbody_batch* batch);
+
+/* Add a system (pos[3n], vel[3n] or NULL, mass[n]); NBODY_ERR_CAPACITY when
+ * the arena has no room.  *id names it until it is removed. */
+nbody_status nbody_batch_add(nbody_batch* batch, const float* pos, const float* vel, const double* mass,
+                             size_t n, int32_t* id);
+nbody_status nbody_batch_remove(nbody_batch* batch, int32_t id);
+
+/* Advance every system nsteps kick-drift steps (asynchronous, one graph launch). */
+nbody_status nbody_batch_step(nbody_batch* batch, float dt, int32_t nsteps);
+
+/* The baseline: each system alone, one single-system launch per step, waiting
+ * for each system before the next.  split_small = 0: the plain launch (one
+ * workgroup per 256 targets, all sources); 1: the single-system default,
+ * which splits a small system's sources over more workgroups. */
+nbody_status nbody_batch_step_sequential(nbody_batch* batch, float dt, int32_t nsteps, int32_t split_small);
+
+nbody_status nbody_batch_synchronize(nbody_batch* batch);
+nbody_status nbody_batch_read(nbody_batch* batch, int32_t id, float* pos, float* vel, float* acc, size_t n);
+nbody_status nbody_batch_get_info(nbody_batch* batch, nbody_batch_info* out);
+nbody_status nbody_batch_set_tuning(nbody_batch* batch, const nbody_batch_tuning* tuning);
+
+/* Time one step of each candidate (targets per lane, block, workgroups per
+ * CU) on the live systems, keep the fastest, and leave the state as it was. */
+nbody_status nbody_batch_tune(nbody_batch* batch, float dt, nbody_batch_tuning* out);
+
+/* Run steps through a captured hipGraph (default) or as plain launches. */
+nbody_status nbody_batch_set_graph(nbody_batch* batch, int32_t enable);
+
+/* Device time of the last step call (events around it) and its interactions. */
+nbody_status nbody_batch_get_timing(nbody_batch* batch, nbody_timing* out);
+
+/* ------------------------------------------------------------------------ */
 /* Names, errors, version                                                    */
 /* ------------------------------------------------------------------------ */
 
@@ -259,6 +332,13 @@ nbody_status nbody_ic_plummer(uint64_t seed, size_t n, double a, double total_ma
 nbody_status nbody_ic_disc_bulge(uint64_t seed, size_t n, double Rd, double z0,
                                  double bulge_fraction, double ab, double total_mass, double G,
                                  float* pos, float* vel, double* mass);
+
+/* Two bulgeless exponential discs of n/2 particles and unit mass each, the
+ * second tilted by tilt_deg about the x axis, on a parabolic relative orbit
+ * at the given separation (along x) with the given pericentre. */
+nbody_status nbody_ic_disc_encounter(uint64_t seed, size_t n, double Rd, double z0, double separation,
+                                     double pericentre, double tilt_deg, double G, float* pos, float* vel,
+                                     double* mass);
 
 /* Uniform cube [-half, half)^3 with zero velocities and masses uniform in
  * [mass_lo, mass_hi). */

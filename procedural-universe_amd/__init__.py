@@ -158,6 +158,33 @@ class BHStats(ctypes.Structure):
     ]
 
 
+class BatchTuning(ctypes.Structure):
+    _fields_ = [
+        ("i_per_lane", ctypes.c_int32),
+        ("block", ctypes.c_int32),
+        ("wg_per_cu", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
+        ("ms_per_step", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {"i_per_lane": self.i_per_lane, "block": self.block, "wg_per_cu": self.wg_per_cu,
+                "ms_per_step": self.ms_per_step}
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("systems", ctypes.c_int32),
+        ("work_items", ctypes.c_int32),
+        ("particles", ctypes.c_uint64),
+        ("interactions_per_step", ctypes.c_uint64),
+        ("capacity", ctypes.c_uint64),
+        ("free_records", ctypes.c_uint64),
+        ("largest_free", ctypes.c_uint64),
+        ("tuning", BatchTuning),
+    ]
+
+
 class NBodyError(RuntimeError):
     def __init__(self, func: str, status: int, message: str):
         self.status = status
@@ -192,6 +219,19 @@ _SIGNATURES = {
         ctypes.c_int,
         [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32), ctypes.c_size_t],
     ),
+    "nbody_batch_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "nbody_batch_destroy": (ctypes.c_int, [_P]),
+    "nbody_batch_add": (ctypes.c_int, [_P, _F32P, _F32P, _F64P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int32)]),
+    "nbody_batch_remove": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "nbody_batch_step": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_int32]),
+    "nbody_batch_step_sequential": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_int32, ctypes.c_int32]),
+    "nbody_batch_synchronize": (ctypes.c_int, [_P]),
+    "nbody_batch_read": (ctypes.c_int, [_P, ctypes.c_int32, _F32P, _F32P, _F32P, ctypes.c_size_t]),
+    "nbody_batch_get_info": (ctypes.c_int, [_P, ctypes.POINTER(BatchInfo)]),
+    "nbody_batch_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(BatchTuning)]),
+    "nbody_batch_tune": (ctypes.c_int, [_P, ctypes.c_float, ctypes.POINTER(BatchTuning)]),
+    "nbody_batch_set_graph": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "nbody_batch_get_timing": (ctypes.c_int, [_P, ctypes.POINTER(Timing)]),
     "nbody_method_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "nbody_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "nbody_last_error": (ctypes.c_char_p, []),
@@ -216,6 +256,11 @@ _SIGNATURES = {
             _F32P,
             _F64P,
         ],
+    ),
+    "nbody_field": (ctypes.c_int, [_P, _F32P, ctypes.c_size_t, _F32P]),
+    "nbody_ic_disc_encounter": (
+        ctypes.c_int,
+        [ctypes.c_uint64, ctypes.c_size_t] + [ctypes.c_double] * 6 + [_F32P, _F32P, _F64P],
     ),
     "nbody_ic_uniform": (
         ctypes.c_int,
@@ -378,6 +423,13 @@ class NBodySim:
         _check("nbody_compute_accelerations", self._lib.nbody_compute_accelerations(self._h, _f32(acc), self.n))
         return acc
 
+    def field(self, points) -> np.ndarray:
+        """Direct-sum acceleration at arbitrary points from the current particles."""
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        out = np.empty_like(pts)
+        _check("nbody_field", self._lib.nbody_field(self._h, _f32(pts), pts.shape[0], _f32(out)))
+        return out
+
     def diagnostics(self) -> dict:
         d = Diag()
         _check("nbody_diagnostics", self._lib.nbody_diagnostics(self._h, ctypes.byref(d)))
@@ -408,6 +460,95 @@ class NBodySim:
     def timing(self) -> Timing:
         t = Timing()
         _check("nbody_get_timing", self._lib.nbody_get_timing(self._h, ctypes.byref(t)))
+        return t
+
+
+class Batch:
+    """Many independent systems advanced together (include/nbody.h, nbody_batch_*)."""
+
+    def __init__(self, config: Config, capacity: int):
+        lib = load_library()
+        self._lib = lib
+        self.config = config
+        h = _P()
+        _check("nbody_batch_create", lib.nbody_batch_create(ctypes.byref(config), int(capacity), ctypes.byref(h)))
+        self._h = h
+        self.sizes = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nbody_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add(self, pos, vel, mass) -> int:
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        mass = np.ascontiguousarray(mass, dtype=np.float64).reshape(n)
+        velp = None
+        if vel is not None:
+            vel = np.ascontiguousarray(vel, dtype=np.float32).reshape(n, 3)
+            velp = _f32(vel)
+        sid = ctypes.c_int32()
+        _check("nbody_batch_add", self._lib.nbody_batch_add(self._h, _f32(pos), velp, _f64(mass), n, ctypes.byref(sid)))
+        self.sizes[sid.value] = n
+        return sid.value
+
+    def remove(self, sid: int):
+        _check("nbody_batch_remove", self._lib.nbody_batch_remove(self._h, int(sid)))
+        self.sizes.pop(sid, None)
+
+    def step(self, dt: float, nsteps: int = 1):
+        _check("nbody_batch_step", self._lib.nbody_batch_step(self._h, float(dt), int(nsteps)))
+
+    def step_sequential(self, dt: float, nsteps: int = 1, split_small: bool = False):
+        _check(
+            "nbody_batch_step_sequential",
+            self._lib.nbody_batch_step_sequential(self._h, float(dt), int(nsteps), 1 if split_small else 0),
+        )
+
+    def synchronize(self):
+        _check("nbody_batch_synchronize", self._lib.nbody_batch_synchronize(self._h))
+
+    def read(self, sid: int):
+        n = self.sizes[sid]
+        pos = np.empty((n, 3), np.float32)
+        vel = np.empty((n, 3), np.float32)
+        acc = np.empty((n, 3), np.float32)
+        _check("nbody_batch_read", self._lib.nbody_batch_read(self._h, int(sid), _f32(pos), _f32(vel), _f32(acc), n))
+        return pos, vel, acc
+
+    def info(self) -> BatchInfo:
+        inf = BatchInfo()
+        _check("nbody_batch_get_info", self._lib.nbody_batch_get_info(self._h, ctypes.byref(inf)))
+        return inf
+
+    def set_tuning(self, i_per_lane: int, block: int, wg_per_cu: int):
+        t = BatchTuning(i_per_lane, block, wg_per_cu)
+        _check("nbody_batch_set_tuning", self._lib.nbody_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def tune(self, dt: float) -> dict:
+        t = BatchTuning()
+        _check("nbody_batch_tune", self._lib.nbody_batch_tune(self._h, float(dt), ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_graph(self, enable: bool):
+        _check("nbody_batch_set_graph", self._lib.nbody_batch_set_graph(self._h, 1 if enable else 0))
+
+    def timing(self) -> Timing:
+        t = Timing()
+        _check("nbody_batch_get_timing", self._lib.nbody_batch_get_timing(self._h, ctypes.byref(t)))
         return t
 
 
@@ -458,6 +599,18 @@ def ic_disc_bulge(
     return pos, vel, mass
 
 
+def ic_disc_encounter(n: int, seed: int, Rd: float = 3.0, z0: float = 0.3, separation: float = 40.0,
+                      pericentre: float = 5.0, tilt_deg: float = 30.0, G: float = 1.0):
+    pos, vel, mass = _ic_out(n)
+    _check(
+        "nbody_ic_disc_encounter",
+        load_library().nbody_ic_disc_encounter(
+            seed, n, Rd, z0, separation, pericentre, tilt_deg, G, _f32(pos), _f32(vel), _f64(mass)
+        ),
+    )
+    return pos, vel, mass
+
+
 def ic_uniform(n: int, seed: int, half: float = 500.0, mass_lo: float = 1e20, mass_hi: float = 1e30):
     pos, vel, mass = _ic_out(n)
     _check(
@@ -465,6 +618,23 @@ def ic_uniform(n: int, seed: int, half: float = 500.0, mass_lo: float = 1e20, ma
         load_library().nbody_ic_uniform(seed, n, half, mass_lo, mass_hi, _f32(pos), _f32(vel), _f64(mass)),
     )
     return pos, vel, mass
+
+
+def batch_galaxies(count: int, seed: int, n_lo: int = 1024, n_hi: int = 65536):
+    """The mixed-size galaxy set: sizes log-uniform in [n_lo, n_hi], each a Plummer
+    sphere or an exponential disc with probability 1/2, seeded per galaxy."""
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(n_lo * (n_hi / n_lo) ** rng.random(count)).astype(np.int64)
+    kinds = rng.random(count) < 0.5
+    out = []
+    for k in range(count):
+        n = int(sizes[k])
+        if kinds[k]:
+            p, v, m = ic_plummer(n, seed * 100003 + k)
+        else:
+            p, v, m = ic_disc_bulge(n, seed * 100003 + k, bulge_fraction=0.0)
+        out.append((p, v, m))
+    return out
 
 
 def particles_from_soa(pos, vel, mass) -> np.ndarray:

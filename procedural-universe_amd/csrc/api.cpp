@@ -418,6 +418,40 @@ nbody_status nbody_compute_accelerations(nbody_sim* s, float* acc, size_t n) {
     return nbody_read_soa(s, nullptr, nullptr, acc, n);
 }
 
+nbody_status nbody_field(nbody_sim* s, const float* points, size_t m, float* acc) {
+    clear_error();
+    NB_CHECK(s, NBODY_ERR_INVALID_ARG, "sim is null");
+    NB_CHECK(s->initialised, NBODY_ERR_NOT_INITIALISED, "nbody_field before nbody_init");
+    NB_CHECK((points && acc) || m == 0, NBODY_ERR_INVALID_ARG, "null argument");
+    NB_CHECK(m <= (size_t(1) << 30), NBODY_ERR_CAPACITY, "too many points");
+    if (m == 0) return NBODY_OK;
+    NB_HIP(hipSetDevice(s->cfg.device));
+    std::vector<float4> h(m);
+    for (size_t i = 0; i < m; ++i) h[i] = make_float4(points[3 * i], points[3 * i + 1], points[3 * i + 2], 0.f);
+    float4* d = nullptr;
+    NB_HIP(hipMalloc(reinterpret_cast<void**>(&d), 2 * m * sizeof(float4)));
+    nbody_status st = NBODY_OK;
+    hipError_t e = hipMemcpyAsync(d, h.data(), m * sizeof(float4), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+        e = direct_field(s->pos[s->cur], static_cast<int>(s->n_pad), d, static_cast<int>(m), s->cfg.law,
+                         static_cast<float>(s->cfg.softening), direct_tiny_for(s->max_gm), d + m, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d + m, m * sizeof(float4), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+        set_error("nbody_field: %s", hipGetErrorString(e));
+        st = NBODY_ERR_DEVICE;
+    }
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipFree(d);
+    if (st != NBODY_OK) return st;
+    for (size_t i = 0; i < m; ++i) {
+        acc[3 * i] = h[i].x;
+        acc[3 * i + 1] = h[i].y;
+        acc[3 * i + 2] = h[i].z;
+    }
+    return NBODY_OK;
+}
+
 nbody_status nbody_size(nbody_sim* s, size_t* n) {
     clear_error();
     NB_CHECK(s && n, NBODY_ERR_INVALID_ARG, "null argument");
@@ -575,6 +609,17 @@ nbody_status nbody_ic_disc_bulge(uint64_t seed, size_t n, double Rd, double z0, 
     NB_CHECK(Rd > 0 && z0 > 0 && bulge_fraction >= 0 && bulge_fraction <= 1 && ab > 0 && total_mass > 0 && G >= 0,
              NBODY_ERR_INVALID_ARG, "bad disc/bulge parameters");
     ic_disc_bulge(seed, n, Rd, z0, bulge_fraction, ab, total_mass, G, pos, vel, mass);
+    return NBODY_OK;
+}
+
+nbody_status nbody_ic_disc_encounter(uint64_t seed, size_t n, double Rd, double z0, double separation,
+                                     double pericentre, double tilt_deg, double G, float* pos, float* vel,
+                                     double* mass) {
+    clear_error();
+    NB_CHECK((pos && vel && mass) || n == 0, NBODY_ERR_INVALID_ARG, "null output");
+    NB_CHECK(n >= 2 && Rd > 0 && z0 > 0 && separation > 0 && pericentre > 0 && pericentre <= separation && G > 0,
+             NBODY_ERR_INVALID_ARG, "bad encounter parameters");
+    ic_disc_encounter(seed, n, Rd, z0, separation, pericentre, tilt_deg, G, pos, vel, mass);
     return NBODY_OK;
 }
 

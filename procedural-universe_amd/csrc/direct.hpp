@@ -53,4 +53,9 @@ DirectShape direct_default_shape(int n_pad);
 hipError_t direct_step(const DirectLaunch& L, const DirectShape& s, bool kick, hipStream_t stream,
                        hipEvent_t ev_force_begin, hipEvent_t ev_force_end);
 
+// Accelerations at m points tgt[] (float4, w ignored) from the n_src_pad
+// sources src[] (padded with massless dummies): out[i], float4 (ax, ay, az, 0).
+hipError_t direct_field(const float4* src, int n_src_pad, const float4* tgt, int m, int law, float soft, float tiny,
+                        float4* out, hipStream_t stream);
+
 }  // namespace nbody

@@ -213,4 +213,34 @@ void ic_uniform(uint64_t seed, size_t n, double half, double mass_lo, double mas
 
 void ic_recentre(size_t n, float* pos, float* vel, const double* mass) { recentre(n, pos, vel, mass); }
 
+void ic_disc_encounter(uint64_t seed, size_t n, double Rd, double z0, double separation, double pericentre,
+                       double tilt_deg, double G, float* pos, float* vel, double* mass) {
+    const size_t n1 = n / 2, n2 = n - n1;
+    // Two bulgeless discs of unit mass, each centred on its own centre of mass.
+    ic_disc_bulge(seed, n1, Rd, z0, 0.0, 1.0, 1.0, G, pos, vel, mass);
+    ic_disc_bulge(seed ^ 0x5A5A5A5A00000000ULL, n2, Rd, z0, 0.0, 1.0, 1.0, G, pos + 3 * n1, vel + 3 * n1, mass + n1);
+    // Tilt the second disc's spin by rotating it about the x axis.
+    const double a = tilt_deg * kPi / 180.0, c = std::cos(a), sn = std::sin(a);
+    for (size_t i = n1; i < n; ++i) {
+        const double y = pos[3 * i + 1], z = pos[3 * i + 2], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+        pos[3 * i + 1] = static_cast<float>(c * y - sn * z);
+        pos[3 * i + 2] = static_cast<float>(sn * y + c * z);
+        vel[3 * i + 1] = static_cast<float>(c * vy - sn * vz);
+        vel[3 * i + 2] = static_cast<float>(sn * vy + c * vz);
+    }
+    // Parabolic relative orbit of total mass M = 2: speed sqrt(2 G M / r0) at
+    // separation r0, and angular momentum sqrt(2 G M q) for pericentre q, so the
+    // velocity makes an angle with the separation whose sine is sqrt(q / r0).
+    const double M = 2.0, r0 = separation;
+    const double v0 = std::sqrt(2.0 * G * M / r0);
+    const double sa = std::sqrt(std::min(1.0, pericentre / r0)), ca = std::sqrt(1.0 - sa * sa);
+    const double rx = r0, vx = -v0 * ca, vy = v0 * sa;  // galaxy 2 relative to galaxy 1
+    for (size_t i = 0; i < n; ++i) {
+        const double h = (i < n1) ? -0.5 : 0.5;  // equal masses: each moves half the relative orbit
+        pos[3 * i] = static_cast<float>(pos[3 * i] + h * rx);
+        vel[3 * i] = static_cast<float>(vel[3 * i] + h * vx);
+        vel[3 * i + 1] = static_cast<float>(vel[3 * i + 1] + h * vy);
+    }
+}
+
 }  // namespace nbody

@@ -13,5 +13,7 @@ void ic_disc_bulge(uint64_t seed, size_t n, double Rd, double z0, double bulge_f
 void ic_uniform(uint64_t seed, size_t n, double half, double mass_lo, double mass_hi, float* pos, float* vel,
                 double* mass);
 void ic_recentre(size_t n, float* pos, float* vel, const double* mass);
+void ic_disc_encounter(uint64_t seed, size_t n, double Rd, double z0, double separation, double pericentre,
+                       double tilt_deg, double G, float* pos, float* vel, double* mass);
 
 }  // namespace nbody
