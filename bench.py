@@ -260,6 +260,109 @@ def bench_bh(pu, args):
     return result
 
 
+def bench_serve(pu, args):
+    """Served workload: Poisson arrivals (rate 200/s for `duration` s), sizes
+    log-uniform in [2048, 524288], steps uniform in [10, 200]; below 32768
+    particles direct sum, above Barnes-Hut theta = 0.6; eps = 0.02."""
+    rng = np.random.default_rng(args.seed)
+    arrivals = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / args.rate)
+        if t > args.duration:
+            break
+        arrivals.append(t)
+    m = len(arrivals)
+    sizes = np.rint(2048 * (524288 / 2048) ** rng.random(m)).astype(np.int64)
+    steps = rng.integers(10, 201, m)
+    cfg = pu.server_config(law=pu.LAW_R, softening=args.eps ** 2, theta=0.6, bh_threshold=32768,
+                           memory_budget=int(args.memory_gib * (1 << 30)), direct_capacity=64 << 20,
+                           max_active_bh=args.max_bh, dt=args.dt)
+    srv = pu.Server(cfg)
+    offered_direct = float(sum(int(n) * int(n) * int(s) for n, s in zip(sizes, steps) if n < 32768))
+    t0 = srv.clock()
+    i = 0
+    results = []
+    timeline = []
+    progress = open(args.progress, "a") if args.progress else None
+    last_report = -1e9
+    while True:
+        now = srv.clock() - t0
+        while i < m and arrivals[i] <= now:
+            srv.submit_plummer(int(sizes[i]), int(steps[i]), args.seed * 1000003 + i, arrival=t0 + arrivals[i])
+            i += 1
+        st = srv.stats()
+        if i == m and st["pending"] == 0 and st["active_direct"] + st["active_bh"] == 0:
+            break
+        if now > args.max_seconds:
+            break
+        if st["pending"] == 0 and st["active_direct"] + st["active_bh"] == 0:
+            time.sleep(min(0.005, max(0.0, arrivals[i] - now)))
+            continue
+        srv.tick()
+        results += srv.poll()
+        if len(timeline) == 0 or now - timeline[-1][0] >= 1.0:
+            timeline.append((now, st["pending"], st["active_direct"], st["active_bh"], st["memory_used"]))
+        if now - last_report >= 2.0:
+            last_report = now
+            line = (f"serve t={now:.1f}s submitted={i}/{m} pending={st['pending']} active_direct={st['active_direct']} "
+                    f"active_bh={st['active_bh']} completed={len(results)} ticks={st['ticks']}")
+            print(line, file=sys.stderr, flush=True)
+            if progress:
+                progress.write(line + "\n")
+                progress.flush()
+    results += srv.poll()
+    st = srv.stats()
+    wall = srv.clock() - t0
+    srv.close()
+    lat = np.array([r["completed"] - r["arrival"] for r in results])
+    direct = [r for r in results if r["method"] == pu.METHOD_DIRECT]
+    bh = [r for r in results if r["method"] == pu.METHOD_BARNES_HUT]
+    inter = float(sum(r["interactions"] for r in results))
+    value = inter / wall
+    pct = (lambda a, q: float(np.percentile(a, q)) if len(a) else None)
+    return {
+        "metric": "served aggregate interactions/s with p50/p99 completion latency (continuous batching)",
+        "value": value,
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": int(st["ticks"]),
+        "warmup": 0,
+        "ms_per_step": 1e3 * st["busy_seconds"] / max(1, st["ticks"]),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"Poisson arrivals {args.rate}/s for {args.duration} s, N log-uniform [2048, 524288] "
+            f"(Plummer spheres generated on the device), steps uniform [10, 200], direct below 32768 else "
+            f"Barnes-Hut theta=0.6, eps={args.eps}, dt={args.dt}",
+            "memory_budget_gib": args.memory_gib,
+            "max_active_bh": args.max_bh,
+        },
+        "latency_s": {
+            "p50": pct(lat, 50), "p99": pct(lat, 99),
+            "p50_direct": pct([r["completed"] - r["arrival"] for r in direct], 50),
+            "p99_direct": pct([r["completed"] - r["arrival"] for r in direct], 99),
+            "p50_bh": pct([r["completed"] - r["arrival"] for r in bh], 50),
+            "p99_bh": pct([r["completed"] - r["arrival"] for r in bh], 99),
+        },
+        "load": {
+            "offered_requests": m,
+            "submitted": int(st["submitted"]),
+            "completed": len(results),
+            "completed_direct": len(direct),
+            "completed_bh": len(bh),
+            "offered_direct_interactions": offered_direct,
+            "completed_interactions": inter,
+            "wall_seconds": wall,
+            "memory_peak_gib": st["memory_peak"] / 2 ** 30,
+            "timeline_s_pending_activeD_activeBH_mem": timeline[:: max(1, len(timeline) // 20)],
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,7 +377,13 @@ def main():
     ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--accuracy-rows", type=int, default=1024)
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU baseline sample (0 = auto)")
-    ap.add_argument("--workload", choices=["direct", "batch", "bh"], default="direct")
+    ap.add_argument("--workload", choices=["direct", "batch", "bh", "serve"], default="direct")
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--max-seconds", type=float, default=300.0)
+    ap.add_argument("--memory-gib", type=float, default=96.0)
+    ap.add_argument("--max-bh", type=int, default=64)
+    ap.add_argument("--progress", default="", help="append serve progress lines to this file")
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--subsample", type=int, default=65536)
     ap.add_argument("--galaxies", type=int, default=512)
@@ -305,6 +414,16 @@ def main():
         if "--seed" not in sys.argv:
             args.seed = 0x5EED0004
         print(json.dumps(bench_batch(pu, args)))
+        return
+
+    if args.workload == "serve":
+        if "--eps" not in sys.argv:
+            args.eps = 0.02
+        if "--dt" not in sys.argv:
+            args.dt = 1e-3
+        if "--seed" not in sys.argv:
+            args.seed = 0x5EED0005
+        print(json.dumps(bench_serve(pu, args)))
         return
 
     if args.workload == "bh":

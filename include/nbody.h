@@ -169,6 +169,11 @@ nbody_status nbody_read(nbody_sim* sim, nbody_particle* particles, size_t n, uin
  * acceleration evaluated (3n floats). */
 nbody_status nbody_read_soa(nbody_sim* sim, float* pos, float* vel, float* acc, size_t n);
 
+/* Initialise with a Plummer sphere (scale a, total mass M, n equal masses,
+ * recentred) generated in device memory from the same counter-based stream as
+ * nbody_ic_plummer, so nothing crosses PCIe. */
+nbody_status nbody_init_plummer(nbody_sim* sim, uint64_t seed, size_t n, double a, double total_mass);
+
 /* Evaluate accelerations at the current positions without stepping; acc
  * receives 3n floats.  Used for parity checks. */
 nbody_status nbody_compute_accelerations(nbody_sim* sim, float* acc, size_t n);
@@ -278,6 +283,11 @@ nbody_status nbody_batch_add(nbody_batch* batch, const float* pos, const float* 
                              size_t n, int32_t* id);
 nbody_status nbody_batch_remove(nbody_batch* batch, int32_t id);
 
+/* Add a Plummer sphere (scale a, total mass M, n equal masses) generated in
+ * the arena by the device from the counter-based stream of nbody_ic_plummer. */
+nbody_status nbody_batch_add_plummer(nbody_batch* batch, uint64_t seed, size_t n, double a, double total_mass,
+                                     int32_t* id);
+
 /* Advance every system nsteps kick-drift steps (asynchronous, one graph launch). */
 nbody_status nbody_batch_step(nbody_batch* batch, float dt, int32_t nsteps);
 
@@ -301,6 +311,86 @@ nbody_status nbody_batch_set_graph(nbody_batch* batch, int32_t enable);
 
 /* Device time of the last step call (events around it) and its interactions. */
 nbody_status nbody_batch_get_timing(nbody_batch* batch, nbody_timing* out);
+
+/* ------------------------------------------------------------------------ */
+/* Request server: continuous batching of galaxy simulations on one GPU       */
+/* ------------------------------------------------------------------------ */
+
+/* Requests (a galaxy's size, seed and step count) are admitted in arrival
+ * order as their device-memory footprint fits the budget; their states are
+ * generated on the device and stay there.  Requests below bh_threshold join
+ * the batch arena (one persistent graph launch per tick); larger ones run
+ * Barnes-Hut on handles of their own, on their own streams, overlapping the
+ * batch on the device.  Each tick advances every active request one step;
+ * requests that finish leave and free their memory, and requests that arrived
+ * meanwhile are admitted on the next tick.  A request's result is the same
+ * alone and in the mix.  Not thread-safe. */
+typedef struct nbody_server nbody_server;
+
+typedef struct nbody_server_config {
+    nbody_config sim;          /* law, G, softening, scale, theta, device (method ignored) */
+    uint64_t memory_budget;    /* device bytes the request states may hold                  */
+    uint64_t direct_capacity;  /* particles in the direct-sum arena                         */
+    uint32_t bh_threshold;     /* requests with n >= bh_threshold use Barnes-Hut            */
+    int32_t  max_active_bh;    /* concurrent Barnes-Hut requests                            */
+    float    dt;               /* step of every request                                    */
+    int32_t  keep_results;     /* keep final states for nbody_server_fetch                 */
+    int32_t  reserved[8];
+} nbody_server_config;
+
+typedef struct nbody_request_result {
+    uint64_t id;
+    uint32_t n;
+    int32_t  steps;
+    int32_t  method;           /* nbody_method                                    */
+    int32_t  pad_;
+    double   arrival;          /* server clock (seconds) at arrival, admission... */
+    double   admitted;
+    double   completed;
+    uint64_t interactions;     /* pair (direct) or counted (Barnes-Hut)           */
+} nbody_request_result;
+
+typedef struct nbody_server_stats {
+    uint64_t submitted;
+    uint64_t admitted;
+    uint64_t completed;
+    uint64_t pending;
+    uint64_t active_direct;
+    uint64_t active_bh;
+    uint64_t idle_handles;
+    uint64_t memory_used;      /* bytes held by admitted requests and idle handles */
+    uint64_t memory_peak;
+    uint64_t interactions;     /* of completed requests                           */
+    uint64_t ticks;
+    double   busy_seconds;     /* time spent in ticks with work                   */
+} nbody_server_stats;
+
+/* law R, eps = 0.02, theta = 0.6, 64 GiB budget, 64 Mi-particle arena,
+ * Barnes-Hut from 32768 particles, at most 64 Barnes-Hut requests, dt = 1e-3. */
+void nbody_server_config_default(nbody_server_config* cfg);
+nbody_status nbody_server_create(const nbody_server_config* cfg, nbody_server** out);
+nbody_status nbody_server_destroy(nbody_server* server);
+
+/* The server's clock: seconds since nbody_server_create. */
+double nbody_server_clock(nbody_server* server);
+
+/* Queue a Plummer-sphere request; arrival is its time on the server clock
+ * (a non-finite value means now). */
+nbody_status nbody_server_submit_plummer(nbody_server* server, uint64_t seed, size_t n, int32_t steps, double a,
+                                         double total_mass, double arrival, uint64_t* id);
+
+/* Admit, advance every active request one step, retire; *completed = requests
+ * that finished in this tick. */
+nbody_status nbody_server_tick(nbody_server* server, int32_t* completed);
+
+/* Take up to max finished-request records (oldest first). */
+nbody_status nbody_server_poll(nbody_server* server, nbody_request_result* out, int32_t max, int32_t* count);
+
+/* Final positions and velocities (3n floats each) of a finished request when
+ * keep_results is set; the kept copy is released. */
+nbody_status nbody_server_fetch(nbody_server* server, uint64_t id, float* pos, float* vel, size_t n);
+
+nbody_status nbody_server_get_stats(nbody_server* server, nbody_server_stats* out);
 
 /* ------------------------------------------------------------------------ */
 /* Names, errors, version                                                    */

@@ -185,6 +185,53 @@ class BatchInfo(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("sim", Config),
+        ("memory_budget", ctypes.c_uint64),
+        ("direct_capacity", ctypes.c_uint64),
+        ("bh_threshold", ctypes.c_uint32),
+        ("max_active_bh", ctypes.c_int32),
+        ("dt", ctypes.c_float),
+        ("keep_results", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64),
+        ("n", ctypes.c_uint32),
+        ("steps", ctypes.c_int32),
+        ("method", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
+        ("arrival", ctypes.c_double),
+        ("admitted", ctypes.c_double),
+        ("completed", ctypes.c_double),
+        ("interactions", ctypes.c_uint64),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("admitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("pending", ctypes.c_uint64),
+        ("active_direct", ctypes.c_uint64),
+        ("active_bh", ctypes.c_uint64),
+        ("idle_handles", ctypes.c_uint64),
+        ("memory_used", ctypes.c_uint64),
+        ("memory_peak", ctypes.c_uint64),
+        ("interactions", ctypes.c_uint64),
+        ("ticks", ctypes.c_uint64),
+        ("busy_seconds", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class NBodyError(RuntimeError):
     def __init__(self, func: str, status: int, message: str):
         self.status = status
@@ -232,6 +279,24 @@ _SIGNATURES = {
     "nbody_batch_tune": (ctypes.c_int, [_P, ctypes.c_float, ctypes.POINTER(BatchTuning)]),
     "nbody_batch_set_graph": (ctypes.c_int, [_P, ctypes.c_int32]),
     "nbody_batch_get_timing": (ctypes.c_int, [_P, ctypes.POINTER(Timing)]),
+    "nbody_init_plummer": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_double, ctypes.c_double]),
+    "nbody_batch_add_plummer": (
+        ctypes.c_int,
+        [_P, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int32)],
+    ),
+    "nbody_server_config_default": (None, [ctypes.POINTER(ServerConfig)]),
+    "nbody_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]),
+    "nbody_server_destroy": (ctypes.c_int, [_P]),
+    "nbody_server_clock": (ctypes.c_double, [_P]),
+    "nbody_server_submit_plummer": (
+        ctypes.c_int,
+        [_P, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+         ctypes.POINTER(ctypes.c_uint64)],
+    ),
+    "nbody_server_tick": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32)]),
+    "nbody_server_poll": (ctypes.c_int, [_P, ctypes.POINTER(RequestResult), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+    "nbody_server_fetch": (ctypes.c_int, [_P, ctypes.c_uint64, _F32P, _F32P, ctypes.c_size_t]),
+    "nbody_server_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(ServerStats)]),
     "nbody_method_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "nbody_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "nbody_last_error": (ctypes.c_char_p, []),
@@ -401,6 +466,12 @@ class NBodySim:
         self._particles = None
         self.n = n
 
+    def init_plummer(self, n: int, seed: int, a: float = 1.0, total_mass: float = 1.0):
+        """Plummer sphere generated in device memory (same stream as ic_plummer)."""
+        _check("nbody_init_plummer", self._lib.nbody_init_plummer(self._h, int(seed), int(n), float(a), float(total_mass)))
+        self._particles = None
+        self.n = n
+
     def step(self, dt: float, nsteps: int = 1):
         _check("nbody_step", self._lib.nbody_step(self._h, float(dt), int(nsteps)))
 
@@ -505,6 +576,15 @@ class Batch:
         self.sizes[sid.value] = n
         return sid.value
 
+    def add_plummer(self, n: int, seed: int, a: float = 1.0, total_mass: float = 1.0) -> int:
+        sid = ctypes.c_int32()
+        _check(
+            "nbody_batch_add_plummer",
+            self._lib.nbody_batch_add_plummer(self._h, int(seed), int(n), float(a), float(total_mass), ctypes.byref(sid)),
+        )
+        self.sizes[sid.value] = n
+        return sid.value
+
     def remove(self, sid: int):
         _check("nbody_batch_remove", self._lib.nbody_batch_remove(self._h, int(sid)))
         self.sizes.pop(sid, None)
@@ -550,6 +630,87 @@ class Batch:
         t = Timing()
         _check("nbody_batch_get_timing", self._lib.nbody_batch_get_timing(self._h, ctypes.byref(t)))
         return t
+
+
+def server_config(**overrides) -> ServerConfig:
+    c = ServerConfig()
+    load_library().nbody_server_config_default(ctypes.byref(c))
+    for k, v in overrides.items():
+        if k == "sim":
+            c.sim = v
+        elif hasattr(c.sim, k) and not hasattr(c, k):
+            setattr(c.sim, k, v)
+        else:
+            setattr(c, k, v)
+    return c
+
+
+class Server:
+    """Continuous-batching request server (include/nbody.h, nbody_server_*)."""
+
+    def __init__(self, config: ServerConfig):
+        lib = load_library()
+        self._lib = lib
+        self.config = config
+        h = _P()
+        _check("nbody_server_create", lib.nbody_server_create(ctypes.byref(config), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nbody_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def clock(self) -> float:
+        return float(self._lib.nbody_server_clock(self._h))
+
+    def submit_plummer(self, n: int, steps: int, seed: int, a: float = 1.0, total_mass: float = 1.0,
+                       arrival: float = float("nan")) -> int:
+        rid = ctypes.c_uint64()
+        _check(
+            "nbody_server_submit_plummer",
+            self._lib.nbody_server_submit_plummer(self._h, int(seed), int(n), int(steps), float(a), float(total_mass),
+                                                  float(arrival), ctypes.byref(rid)),
+        )
+        return rid.value
+
+    def tick(self) -> int:
+        c = ctypes.c_int32()
+        _check("nbody_server_tick", self._lib.nbody_server_tick(self._h, ctypes.byref(c)))
+        return c.value
+
+    def poll(self, max_results: int = 4096):
+        buf = (RequestResult * max_results)()
+        c = ctypes.c_int32()
+        _check("nbody_server_poll", self._lib.nbody_server_poll(self._h, buf, max_results, ctypes.byref(c)))
+        return [
+            {"id": r.id, "n": r.n, "steps": r.steps, "method": r.method, "arrival": r.arrival, "admitted": r.admitted,
+             "completed": r.completed, "interactions": r.interactions}
+            for r in buf[: c.value]
+        ]
+
+    def fetch(self, rid: int, n: int):
+        pos = np.empty((n, 3), np.float32)
+        vel = np.empty((n, 3), np.float32)
+        _check("nbody_server_fetch", self._lib.nbody_server_fetch(self._h, int(rid), _f32(pos), _f32(vel), n))
+        return pos, vel
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check("nbody_server_get_stats", self._lib.nbody_server_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
 
 
 def create_nbody_sim(kind, config: Optional[Config] = None) -> NBodySim:

@@ -21,6 +21,7 @@
 #include "diag.hpp"
 #include "direct.hpp"
 #include "ic.hpp"
+#include "ic_device.hpp"
 #include "sim.hpp"
 
 static_assert(sizeof(nbody_particle) == 104, "nbody_particle must match the reference's 104-byte Particle");
@@ -99,8 +100,9 @@ nbody_status release_state(nbody_sim* s) {
     free_dev(s->mass_dev);
     free_dev(s->gm_dev);
     free_dev(s->diag_buf);
+    free_dev(s->ic_scratch);
     s->pos[0] = s->pos[1] = s->vel = s->acc = s->part = nullptr;
-    s->mass_dev = s->gm_dev = s->diag_buf = nullptr;
+    s->mass_dev = s->gm_dev = s->diag_buf = s->ic_scratch = nullptr;
     s->capacity = 0;
     s->part_elems = 0;
     s->diag_elems = 0;
@@ -115,9 +117,9 @@ DirectShape shape_for(const nbody_sim* s) {
     return d;
 }
 
-// Upload a packed state (n real records) into freshly sized device buffers.
-nbody_status upload_state(nbody_sim* s, const std::vector<float4>& pos, const std::vector<float4>& vel,
-                          const std::vector<double>& mass, size_t n) {
+// Size the device buffers for n particles (keeping larger ones) and set the
+// launch shape; the contents are left to the caller.
+nbody_status prepare_state(nbody_sim* s, size_t n) {
     NB_HIP(hipSetDevice(s->cfg.device));
     NB_HIP(hipStreamSynchronize(s->stream));
     const size_t n_pad = round_up(std::max<size_t>(n, 1), kTile);
@@ -141,6 +143,14 @@ nbody_status upload_state(nbody_sim* s, const std::vector<float4>& pos, const st
         NB_TRY(dev_alloc(&s->part, need_part));
         s->part_elems = need_part;
     }
+    return NBODY_OK;
+}
+
+// Upload a packed state (n real records) into freshly sized device buffers.
+nbody_status upload_state(nbody_sim* s, const std::vector<float4>& pos, const std::vector<float4>& vel,
+                          const std::vector<double>& mass, size_t n) {
+    NB_TRY(prepare_state(s, n));
+    const size_t n_pad = s->n_pad;
     std::vector<float4> hp(n_pad, make_float4(0.f, 0.f, 0.f, 0.f)), hv(n_pad, make_float4(0.f, 0.f, 0.f, 0.f));
     std::vector<double> hm(n_pad, 0.0), hg(n_pad, 0.0);
     std::copy(pos.begin(), pos.begin() + n, hp.begin());
@@ -316,6 +326,34 @@ nbody_status nbody_init_soa(nbody_sim* s, const float* pos3, const float* vel3, 
         m[i] = mass[i];
     }
     return upload_state(s, pos, vel, m, n);
+}
+
+nbody_status nbody_init_plummer(nbody_sim* s, uint64_t seed, size_t n, double a, double total_mass) {
+    clear_error();
+    NB_CHECK(s, NBODY_ERR_INVALID_ARG, "sim is null");
+    NB_CHECK(n >= 1 && n <= (size_t(1) << 30), NBODY_ERR_INVALID_ARG, "n = %zu out of range", n);
+    NB_CHECK(a > 0 && total_mass > 0, NBODY_ERR_INVALID_ARG, "bad Plummer parameters");
+    NB_TRY(prepare_state(s, n));
+    if (!s->ic_scratch) NB_TRY(dev_alloc(&s->ic_scratch, ic_device_scratch_doubles()));
+    const size_t bytes = s->n_pad * sizeof(float4);
+    NB_HIP(hipMemsetAsync(s->pos[0], 0, bytes, s->stream));
+    NB_HIP(hipMemsetAsync(s->pos[1], 0, bytes, s->stream));
+    NB_HIP(hipMemsetAsync(s->vel, 0, bytes, s->stream));
+    NB_HIP(hipMemsetAsync(s->acc, 0, bytes, s->stream));
+    NB_HIP(hipMemsetAsync(s->mass_dev, 0, s->n_pad * sizeof(double), s->stream));
+    NB_HIP(ic_device_plummer(seed, static_cast<int>(n), a, total_mass, s->cfg.G, s->pos[0], s->vel, s->mass_dev,
+                             s->pos[1], s->ic_scratch, s->stream));
+    const double m = total_mass / static_cast<double>(n);
+    std::vector<double> hg(s->n_pad, 0.0);
+    std::fill(hg.begin(), hg.begin() + n, s->cfg.G * m);
+    NB_HIP(hipMemcpyAsync(s->gm_dev, hg.data(), s->n_pad * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    NB_HIP(hipStreamSynchronize(s->stream));
+    s->mass_host.assign(n, m);
+    s->max_gm = std::fabs(s->cfg.G * m);
+    s->initialised = true;
+    s->acc_valid = false;
+    if (s->bh) NB_TRY(bh_resize(s->bh, n, s));
+    return NBODY_OK;
 }
 
 nbody_status nbody_step(nbody_sim* s, float dt, int32_t nsteps) {

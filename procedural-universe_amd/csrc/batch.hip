@@ -26,6 +26,7 @@
 #include "common.hpp"
 #include "direct.hpp"
 #include "direct_core.hpp"
+#include "ic_device.hpp"
 #include "sim.hpp"
 
 namespace nbody {
@@ -186,6 +187,8 @@ struct nbody_batch {
     float graph_dt = 0.f;
     int graph_parity = -1;
     int graph_k = 0, graph_block = 0, graph_wg = 0;
+
+    double* ic_scratch = nullptr;  // device initial conditions
 
     // sequential baseline: partial-sum buffer of the plain kernel
     float4* part = nullptr;
@@ -374,7 +377,8 @@ nbody_status nbody_batch_destroy(nbody_batch* b) {
     drop_graph(b);
     for (void* p : {static_cast<void*>(b->pos[0]), static_cast<void*>(b->pos[1]), static_cast<void*>(b->vel),
                     static_cast<void*>(b->acc), static_cast<void*>(b->d_sys), static_cast<void*>(b->d_items),
-                    static_cast<void*>(b->d_n_items), static_cast<void*>(b->d_counters), static_cast<void*>(b->part)})
+                    static_cast<void*>(b->d_n_items), static_cast<void*>(b->d_counters), static_cast<void*>(b->part),
+                    static_cast<void*>(b->ic_scratch)})
         if (p) (void)hipFree(p);
     for (hipEvent_t e : b->ev)
         if (e) (void)hipEventDestroy(e);
@@ -426,6 +430,49 @@ nbody_status nbody_batch_add(nbody_batch* b, const float* pos3, const float* vel
     y.n_pad = n_pad;
     y.max_gm = max_gm;
     y.tiny = direct_tiny_for(max_gm);
+    b->items_dirty = true;
+    *id = slot;
+    return NBODY_OK;
+}
+
+nbody_status nbody_batch_add_plummer(nbody_batch* b, uint64_t seed, size_t n, double a, double total_mass,
+                                     int32_t* id) {
+    clear_error();
+    NB_CHECK(b && id, NBODY_ERR_INVALID_ARG, "null argument");
+    NB_CHECK(n > 0 && n <= (size_t(1) << 28) && a > 0 && total_mass > 0, NBODY_ERR_INVALID_ARG,
+             "bad Plummer system (n %zu)", n);
+    NB_HIP(hipSetDevice(b->cfg.device));
+    if (!b->ic_scratch) NB_HIP(hipMalloc(reinterpret_cast<void**>(&b->ic_scratch), ic_device_scratch_doubles() * sizeof(double)));
+    const size_t n_pad = round_up(n, kTile);
+    size_t off = 0;
+    NB_TRY(arena_alloc(b, n_pad, &off));
+    const size_t bytes = n_pad * sizeof(float4);
+    hipError_t e = hipMemsetAsync(b->pos[b->cur] + off, 0, bytes, b->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b->pos[b->cur ^ 1] + off, 0, bytes, b->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b->vel + off, 0, bytes, b->stream);
+    if (e == hipSuccess)
+        e = ic_device_plummer(seed, static_cast<int>(n), a, total_mass, b->cfg.G, b->pos[b->cur] + off, b->vel + off,
+                              nullptr, b->pos[b->cur ^ 1] + off, b->ic_scratch, b->stream);
+    if (e != hipSuccess) {
+        arena_free(b, off, n_pad);
+        set_error("device initial conditions failed: %s", hipGetErrorString(e));
+        return NBODY_ERR_DEVICE;
+    }
+    int slot;
+    if (!b->free_ids.empty()) {
+        slot = b->free_ids.back();
+        b->free_ids.pop_back();
+    } else {
+        slot = static_cast<int>(b->systems.size());
+        b->systems.emplace_back();
+    }
+    auto& y = b->systems[slot];
+    y.live = true;
+    y.off = off;
+    y.n = n;
+    y.n_pad = n_pad;
+    y.max_gm = std::fabs(b->cfg.G * total_mass / static_cast<double>(n));
+    y.tiny = direct_tiny_for(y.max_gm);
     b->items_dirty = true;
     *id = slot;
     return NBODY_OK;

@@ -23,8 +23,9 @@
 //                    than its parent's is an octree cell ("kept"), the others
 //                    are the binary splits inside one cell; identical keys
 //                    form a level-21 bucket;
-//  5. mass           FP64 mass and mass-weighted position summed bottom-up
-//                    (the second child to finish adds both, in a fixed order);
+//  5. mass           FP64 mass and mass-weighted position of every node as a
+//                    difference of double-double prefix sums over the sorted
+//                    particles (a node is a contiguous range of them);
 //  6. traversal      one lane per particle in Morton order; a wave walks the
 //                    octree with one stack in LDS of (cell, lanes) entries.
 //                    Each lane makes the reference's own decision at each cell
@@ -62,7 +63,6 @@ constexpr int kBucketLevel = kLevels;
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
-constexpr int kScanThreads = 1024;
 constexpr int kTravThreads = 256;
 constexpr int kTravWaves = kTravThreads / 64;
 constexpr int kStackCap = 176;  // > 7 * 22 + 8 pending cells per wave
@@ -209,27 +209,72 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const unsigned
     hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of `total` counters in one workgroup.
-__global__ __launch_bounds__(kScanThreads) void radix_scan_kernel(unsigned* __restrict__ hist, int total) {
-    __shared__ unsigned sums[kScanThreads];
+// Exclusive scan of `total` counters: chunks of kScanChunk are reduced in
+// parallel (coalesced), the chunk sums are scanned by one workgroup, then each
+// chunk is scanned and offset.
+constexpr int kScanThreads2 = 256;
+constexpr int kScanPer = 16;
+constexpr int kScanChunk = kScanThreads2 * kScanPer;
+
+__device__ inline unsigned block_exclusive_scan_u32(unsigned v, unsigned* sh, unsigned* total) {
     const int t = threadIdx.x;
-    const int per = (total + kScanThreads - 1) / kScanThreads;
-    const int b = t * per, e = min(b + per, total);
-    unsigned s = 0;
-    for (int i = b; i < e; ++i) s += hist[i];
-    sums[t] = s;
+    sh[t] = v;
     __syncthreads();
-    for (int off = 1; off < kScanThreads; off <<= 1) {
-        const unsigned v = t >= off ? sums[t - off] : 0u;
+    for (int off = 1; off < kScanThreads2; off <<= 1) {
+        const unsigned a = t >= off ? sh[t - off] : 0u;
         __syncthreads();
-        sums[t] += v;
+        sh[t] += a;
         __syncthreads();
     }
-    unsigned run = sums[t] - s;
-    for (int i = b; i < e; ++i) {
-        const unsigned v = hist[i];
-        hist[i] = run;
-        run += v;
+    const unsigned incl = sh[t];
+    if (total) *total = sh[kScanThreads2 - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(kScanThreads2) void scan_reduce_kernel(const unsigned* __restrict__ in, int total,
+                                                                    unsigned* __restrict__ chunk_sums) {
+    __shared__ unsigned sh[kScanThreads2];
+    const int base = blockIdx.x * kScanChunk;
+    unsigned s = 0;
+    for (int k = 0; k < kScanPer; ++k) {
+        const int i = base + k * kScanThreads2 + threadIdx.x;
+        if (i < total) s += in[i];
+    }
+    unsigned tot = 0;
+    block_exclusive_scan_u32(s, sh, &tot);
+    if (threadIdx.x == 0) chunk_sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads2) void scan_chunks_kernel(unsigned* __restrict__ chunk_sums, int nchunks) {
+    __shared__ unsigned sh[kScanThreads2];
+    unsigned carry = 0;
+    for (int b = 0; b < nchunks; b += kScanThreads2) {
+        const int i = b + threadIdx.x;
+        const unsigned v = i < nchunks ? chunk_sums[i] : 0u;
+        unsigned tot = 0;
+        const unsigned ex = block_exclusive_scan_u32(v, sh, &tot);
+        if (i < nchunks) chunk_sums[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads2) void scan_apply_kernel(unsigned* __restrict__ data, int total,
+                                                                   const unsigned* __restrict__ chunk_off) {
+    __shared__ unsigned sh[kScanThreads2];
+    const int base = blockIdx.x * kScanChunk + threadIdx.x * kScanPer;  // contiguous run per thread
+    unsigned v[kScanPer];
+    unsigned s = 0;
+    for (int k = 0; k < kScanPer; ++k) {
+        const int i = base + k;
+        v[k] = i < total ? data[i] : 0u;
+        s += v[k];
+    }
+    unsigned run = chunk_off[blockIdx.x] + block_exclusive_scan_u32(s, sh, nullptr);
+    for (int k = 0; k < kScanPer; ++k) {
+        const int i = base + k;
+        if (i < total) data[i] = run;
+        run += v[k];
     }
 }
 
@@ -378,52 +423,144 @@ __global__ __launch_bounds__(256) void link_kernel(const unsigned long long* __r
 }
 
 // ---- 5. FP64 mass and centre of mass ------------------------------------------------
+//
+// Every radix-tree node covers a contiguous range [first, last] of the sorted
+// particles, so its mass and mass-weighted position are differences of
+// exclusive prefix sums P[last + 1] - P[first] of (m, m x, m y, m z).  The
+// prefix sums are double-double (error-free TwoSum additions, Knuth), so a
+// node's FP64 sums are exact to rounding however small the node is next to
+// the total, with no atomics and no ordering between nodes.
+
+struct DD {
+    double hi, lo;
+};
+
+__device__ inline DD dd_add(DD a, DD b) {
+    const double s = a.hi + b.hi;
+    const double bb = s - a.hi;
+    const double e = (a.hi - (s - bb)) + (b.hi - bb) + a.lo + b.lo;
+    const double hi = s + e;
+    return DD{hi, e - (hi - s)};
+}
+
+__device__ inline double dd_diff(DD a, DD b) {  // a - b rounded to double
+    const DD nb{-b.hi, -b.lo};
+    const DD d = dd_add(a, nb);
+    return d.hi + d.lo;
+}
+
+constexpr int kMassThreads = 256;
+constexpr int kMassPer = 16;
+constexpr int kMassChunk = kMassThreads * kMassPer;
 
 struct MassArgs {
     const float4* pos_sorted;
-    const unsigned* order;   // sorted position -> particle
-    const double* mass;      // by particle
-    const int* left;
-    const int* right;
-    const int* parent;
-    const int* leaf_parent;
-    unsigned* visit;         // zeroed
-    double4* sums;           // [n-1] (M, Mx, My, Mz)
+    const unsigned* order;  // sorted position -> particle
+    const double* mass;     // by particle
     int n;
+    DD* prefix;             // [4][n + 1]
+    DD* chunk;              // [4][nchunks]
+    int nchunks;
 };
 
-__device__ inline double4 leaf_sum(const MassArgs& A, int k) {
+__device__ inline void leaf_vals(const MassArgs& A, int k, double v[4]) {
+    if (k >= A.n) {
+        v[0] = v[1] = v[2] = v[3] = 0.0;
+        return;
+    }
     const double m = A.mass[A.order[k]];
     const float4 p = A.pos_sorted[k];
-    return make_double4(m, m * static_cast<double>(p.x), m * static_cast<double>(p.y), m * static_cast<double>(p.z));
+    v[0] = m;
+    v[1] = m * static_cast<double>(p.x);
+    v[2] = m * static_cast<double>(p.y);
+    v[3] = m * static_cast<double>(p.z);
 }
 
-__global__ __launch_bounds__(256) void mass_kernel(MassArgs A) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= A.n) return;
-    int node = A.leaf_parent[k];
-    while (node >= 0) {
-        __threadfence();
-        const unsigned prev = atomicAdd(&A.visit[node], 1u);
-        if (prev == 0) return;  // the sibling subtree finishes this node
-        __threadfence();
-        const int lc = A.left[node], rc = A.right[node];
-        double4 a, b;
-        if (lc < 0) {
-            a = leaf_sum(A, dec_leaf(lc));
-        } else {
-            volatile const double* s = &A.sums[lc].x;
-            a = make_double4(s[0], s[1], s[2], s[3]);
-        }
-        if (rc < 0) {
-            b = leaf_sum(A, dec_leaf(rc));
-        } else {
-            volatile const double* s = &A.sums[rc].x;
-            b = make_double4(s[0], s[1], s[2], s[3]);
-        }
-        A.sums[node] = make_double4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
-        node = A.parent[node];
+// Block-wide inclusive scan of one DD per thread (Hillis-Steele in LDS).
+__device__ inline DD block_scan_dd(DD x, DD* sh) {
+    const int t = threadIdx.x;
+    sh[t] = x;
+    __syncthreads();
+    for (int off = 1; off < kMassThreads; off <<= 1) {
+        const DD a = t >= off ? sh[t - off] : DD{0.0, 0.0};
+        __syncthreads();
+        sh[t] = dd_add(a, sh[t]);
+        __syncthreads();
     }
+    const DD r = sh[t];
+    __syncthreads();
+    return r;
+}
+
+// Sum of each chunk, per quantity.
+__global__ __launch_bounds__(kMassThreads) void mass_reduce_kernel(MassArgs A) {
+    __shared__ DD sh[kMassThreads];
+    const int base = blockIdx.x * kMassChunk + threadIdx.x * kMassPer;
+    DD acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int k = 0; k < kMassPer; ++k) {
+        double v[4];
+        leaf_vals(A, base + k, v);
+        for (int q = 0; q < 4; ++q) acc[q] = dd_add(acc[q], DD{v[q], 0.0});
+    }
+    for (int q = 0; q < 4; ++q) {
+        const DD incl = block_scan_dd(acc[q], sh);
+        if (threadIdx.x == kMassThreads - 1) A.chunk[q * A.nchunks + blockIdx.x] = incl;
+    }
+}
+
+// Exclusive scan of the chunk sums (one workgroup, quantities in turn).
+__global__ __launch_bounds__(kMassThreads) void mass_chunks_kernel(MassArgs A) {
+    __shared__ DD sh[kMassThreads];
+    for (int q = 0; q < 4; ++q) {
+        DD carry{0.0, 0.0};
+        for (int b = 0; b < A.nchunks; b += kMassThreads) {
+            const int i = b + threadIdx.x;
+            const DD v = i < A.nchunks ? A.chunk[q * A.nchunks + i] : DD{0.0, 0.0};
+            const DD incl = block_scan_dd(v, sh);
+            const DD ex = dd_add(incl, DD{-v.hi, -v.lo});
+            const DD tot = sh[kMassThreads - 1];
+            __syncthreads();
+            if (i < A.nchunks) A.chunk[q * A.nchunks + i] = dd_add(carry, ex);
+            carry = dd_add(carry, tot);
+        }
+    }
+}
+
+// Exclusive prefix P[k] of every particle k, and P[n] (the total).
+__global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
+    __shared__ DD sh[kMassThreads];
+    const int base = blockIdx.x * kMassChunk + threadIdx.x * kMassPer;
+    const size_t stride = static_cast<size_t>(A.n) + 1;
+    for (int q = 0; q < 4; ++q) {
+        DD run{0.0, 0.0};
+        for (int k = 0; k < kMassPer; ++k) {
+            double v[4];
+            leaf_vals(A, base + k, v);
+            run = dd_add(run, DD{v[q], 0.0});
+        }
+        const DD incl = block_scan_dd(run, sh);
+        DD p = dd_add(A.chunk[q * A.nchunks + blockIdx.x], dd_add(incl, DD{-run.hi, -run.lo}));
+        for (int k = 0; k < kMassPer; ++k) {
+            const int i = base + k;
+            if (i <= A.n) A.prefix[q * stride + i] = p;
+            double v[4];
+            leaf_vals(A, i, v);
+            p = dd_add(p, DD{v[q], 0.0});
+        }
+    }
+}
+
+// Node sums from the prefix differences: (M, Mx, My, Mz) in FP64.
+__global__ __launch_bounds__(256) void node_sums_kernel(const DD* __restrict__ prefix, int n,
+                                                        const int* __restrict__ first, const int* __restrict__ last,
+                                                        int m, double4* __restrict__ sums) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const size_t stride = static_cast<size_t>(n) + 1;
+    const int f = first[i], l = last[i] + 1;
+    double r[4];
+    for (int q = 0; q < 4; ++q) r[q] = dd_diff(prefix[q * stride + l], prefix[q * stride + f]);
+    sums[i] = make_double4(r[0], r[1], r[2], r[3]);
 }
 
 // Traversal records of the kept cells: (COM, G*M) in FP32 and the cell side.
@@ -651,7 +788,10 @@ struct BHState {
     int* level = nullptr;
     int* kept = nullptr;
     int* child = nullptr;
-    unsigned* visit = nullptr;
+    unsigned* chunk_sums = nullptr;  // sort scan
+    size_t chunk_elems = 0;
+    DD* prefix = nullptr;            // [4][n + 1] double-double prefix sums
+    DD* mass_chunk = nullptr;        // [4][nchunks]
     double4* sums = nullptr;
     float4* com = nullptr;
     float* side = nullptr;
@@ -689,7 +829,9 @@ void bh_release(BHState* b) {
     bh_free(b->level);
     bh_free(b->kept);
     bh_free(b->child);
-    bh_free(b->visit);
+    bh_free(b->chunk_sums);
+    bh_free(b->prefix);
+    bh_free(b->mass_chunk);
     bh_free(b->sums);
     bh_free(b->com);
     bh_free(b->side);
@@ -808,7 +950,9 @@ nbody_status bh_resize(BHState* b, size_t n, nbody_sim*) {
     NB_TRY(bh_alloc(b->level, m));
     NB_TRY(bh_alloc(b->kept, m));
     NB_TRY(bh_alloc(b->child, 8 * m));
-    NB_TRY(bh_alloc(b->visit, m));
+    NB_TRY(bh_alloc(b->chunk_sums, static_cast<size_t>(blocks_for(256 * nb, kScanChunk))));
+    NB_TRY(bh_alloc(b->prefix, 4 * (n + 1)));
+    NB_TRY(bh_alloc(b->mass_chunk, 4 * static_cast<size_t>(blocks_for(n + 1, kMassChunk))));
     NB_TRY(bh_alloc(b->sums, m));
     NB_TRY(bh_alloc(b->com, m));
     NB_TRY(bh_alloc(b->side, m));
@@ -832,7 +976,10 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     for (int pass = 0; pass < 8; ++pass) {
         const int shift = 8 * pass;
         hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], n, shift, b->hist, nb);
-        hipLaunchKernelGGL(radix_scan_kernel, dim3(1), dim3(kScanThreads), 0, st, b->hist, 256 * nb);
+        const int total = 256 * nb, nchunks = blocks_for(total, kScanChunk);
+        hipLaunchKernelGGL(scan_reduce_kernel, dim3(nchunks), dim3(kScanThreads2), 0, st, b->hist, total, b->chunk_sums);
+        hipLaunchKernelGGL(scan_chunks_kernel, dim3(1), dim3(kScanThreads2), 0, st, b->chunk_sums, nchunks);
+        hipLaunchKernelGGL(scan_apply_kernel, dim3(nchunks), dim3(kScanThreads2), 0, st, b->hist, total, b->chunk_sums);
         hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], b->idx[cur],
                            b->keys[cur ^ 1], b->idx[cur ^ 1], n, shift, b->hist, nb);
         cur ^= 1;
@@ -848,9 +995,13 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
                        b->child);
     hipLaunchKernelGGL(link_kernel, dim3(blocks_for(m + n, 256)), dim3(256), 0, st, keys, n, b->parent,
                        b->leaf_parent, b->first, b->level, b->kept, b->child);
-    NB_HIP(hipMemsetAsync(b->visit, 0, static_cast<size_t>(m) * sizeof(unsigned), st));
-    MassArgs M{b->pos_sorted, order, s->mass_dev, b->left, b->right, b->parent, b->leaf_parent, b->visit, b->sums, n};
-    hipLaunchKernelGGL(mass_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, M);
+    const int mchunks = blocks_for(n + 1, kMassChunk);
+    MassArgs M{b->pos_sorted, order, s->mass_dev, n, b->prefix, b->mass_chunk, mchunks};
+    hipLaunchKernelGGL(mass_reduce_kernel, dim3(mchunks), dim3(kMassThreads), 0, st, M);
+    hipLaunchKernelGGL(mass_chunks_kernel, dim3(1), dim3(kMassThreads), 0, st, M);
+    hipLaunchKernelGGL(mass_prefix_kernel, dim3(mchunks), dim3(kMassThreads), 0, st, M);
+    hipLaunchKernelGGL(node_sums_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->prefix, n, b->first, b->last, m,
+                       b->sums);
     hipLaunchKernelGGL(finalize_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->sums, b->kept, b->level, m,
                        s->cfg.G, b->root, b->com, b->side);
     NB_HIP(hipGetLastError());

@@ -34,6 +34,7 @@ struct nbody_sim {
     double* mass_dev = nullptr;           // FP64 masses (diagnostics)
     double* gm_dev = nullptr;             // FP64 G*m (diagnostics)
     double* diag_buf = nullptr;
+    double* ic_scratch = nullptr;         // device initial conditions: recentring sums
     size_t diag_elems = 0;
     std::vector<double> mass_host;        // FP64 masses (FORCES read-back = m * a)
     double max_gm = 0.0;                  // largest |G*m| (sizes the law-R guard)
