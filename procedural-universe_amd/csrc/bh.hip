@@ -52,6 +52,7 @@
 #include "bh.hpp"
 #include "common.hpp"
 #include "direct.hpp"
+#include "direct_core.hpp"
 #include "sim.hpp"
 
 namespace nbody {
@@ -614,31 +615,7 @@ struct TravArgs {
     float dts;
 };
 
-template <int LAW>
-__device__ __forceinline__ void pair_acc(float sx, float sy, float sz, float gm, float px, float py, float pz,
-                                         float soft, float tiny, float& ax, float& ay, float& az) {
-    const float dx = sx - px, dy = sy - py, dz = sz - pz;
-    float s;
-    if constexpr (LAW == NBODY_LAW_R) {
-        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-        const float t = r2 + soft;
-        s = gm * __builtin_amdgcn_rsqf(__builtin_fmaf(r2 * t, t, tiny));
-    } else {
-        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, soft)));
-        const float inv = __builtin_amdgcn_rsqf(r2);
-        s = (gm * inv) * (inv * inv);
-    }
-    ax = __builtin_fmaf(dx, s, ax);
-    ay = __builtin_fmaf(dy, s, ay);
-    az = __builtin_fmaf(dz, s, az);
-}
-
-__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
-    const float y = x - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-}
+using core::kahan_add;
 
 // The reference's opening test, Octree.cpp:122-124: approximate iff
 // side / |p - COM| < theta, with |p - COM| = sqrtf(dx*dx + dy*dy + dz*dz).  A
@@ -655,25 +632,99 @@ __device__ __forceinline__ bool approximate(float side, float dx, float dy, floa
     return q <= theta_lo;
 }
 
+// The walk and the arithmetic are separated.  The walk (uniform over the
+// wave) decides, per lane and per cell, approximate or open, and appends
+// (source, lanes) entries to a list in LDS: a cell's monopole with the lanes
+// that approximate it, a single particle with the lanes that opened its
+// parent (its own lane removed).  When the list fills, every lane evaluates
+// all entries in a tight loop - four sources at a time with grouped rsq, the
+// source's G*m zeroed for lanes outside the entry - and adds the result to its
+// compensated sum.
+constexpr int kListCap = 128;
+
+__device__ __forceinline__ float lane_gm(float gm, unsigned lo, unsigned hi, int lane) {
+    const unsigned w = lane < 32 ? lo : hi;
+    return ((w >> (lane & 31)) & 1u) ? gm : 0.f;
+}
+
 template <int LAW>
-__global__ __launch_bounds__(kTravThreads) void traverse_kernel(TravArgs A) {
+__device__ __forceinline__ void flush_list(const float4* lsrc, const uint2* lmask, int cnt, int lane, float px,
+                                           float py, float pz, float soft, float tiny, float& sx, float& sy,
+                                           float& sz, float& cx, float& cy, float& cz) {
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    for (int e = 0; e < cnt; e += 4) {
+        float4 src[4];
+        float dx[4], dy[4], dz[4], q[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            src[g] = lsrc[e + g];
+            const uint2 m = lmask[e + g];
+            src[g].w = lane_gm(src[g].w, m.x, m.y, lane);
+            q[g] = core::pair_arg<LAW>(src[g], px, py, pz, soft, tiny, dx[g], dy[g], dz[g]);
+        }
+        core::rsq4(q[0], q[1], q[2], q[3]);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) core::pair_acc<LAW>(src[g].w, q[g], dx[g], dy[g], dz[g], ax, ay, az);
+    }
+    kahan_add(sx, cx, ax);
+    kahan_add(sy, cy, ay);
+    kahan_add(sz, cz, az);
+}
+
+template <int LAW>
+__global__ __launch_bounds__(kTravThreads) void traverse_kernel(
+    const float4* __restrict__ pos_sorted, const unsigned* __restrict__ order, const float4* __restrict__ node_com,
+    const float* __restrict__ node_side, const int* __restrict__ node_level, const int* __restrict__ node_child,
+    const int* __restrict__ node_first, const int* __restrict__ node_last, TravArgs A) {
     __shared__ int stk_node[kTravWaves][kStackCap];
-    __shared__ unsigned long long stk_mask[kTravWaves][kStackCap];
-    __shared__ unsigned long long red[kTravThreads];
+    __shared__ uint2 stk_mask[kTravWaves][kStackCap];
+    __shared__ float4 lst_src[kTravWaves][kListCap];
+    __shared__ uint2 lst_mask[kTravWaves][kListCap];
+    __shared__ unsigned long long red[kTravWaves];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int t = blockIdx.x * kTravThreads + tid;
+    const int wave_base = blockIdx.x * kTravThreads + wave * 64;
     const bool valid = t < A.n;
-    const int me_k = valid ? t : A.n - 1;
-    const float4 me = A.pos_sorted[me_k];
-    float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
-    unsigned long long count = 0;
+    const float4 me = pos_sorted[valid ? t : A.n - 1];
+    const float soft = core::in_vgpr(A.soft);
+    const float tiny = core::in_vgpr(A.tiny);
+    float sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+    unsigned long long count = 0;  // wave-uniform
+    int cnt = 0;                   // list entries (wave-uniform)
+
+    auto append = [&](const float4 src, unsigned long long mask) {
+        if (lane == 0) {
+            lst_src[wave][cnt] = src;
+            lst_mask[wave][cnt] = make_uint2(static_cast<unsigned>(mask), static_cast<unsigned>(mask >> 32));
+        }
+        count += static_cast<unsigned long long>(__popcll(mask));
+        ++cnt;
+    };
+    auto flush = [&]() {
+        if (lane == 0)
+            while (cnt & 3) {
+                lst_src[wave][cnt] = make_float4(0.f, 0.f, 0.f, 0.f);
+                lst_mask[wave][cnt] = make_uint2(0u, 0u);
+                ++cnt;
+            }
+        cnt = (cnt + 3) & ~3;
+        __builtin_amdgcn_wave_barrier();
+        flush_list<LAW>(lst_src[wave], lst_mask[wave], cnt, lane, me.x, me.y, me.z, soft, tiny, sx, sy, sz, cx, cy, cz);
+        __builtin_amdgcn_wave_barrier();
+        cnt = 0;
+    };
+    // the lanes of a leaf's own particle never interact with it
+    auto without_self = [&](unsigned long long m, int q) {
+        const int l = q - wave_base;
+        return (l >= 0 && l < 64) ? (m & ~(1ULL << l)) : m;
+    };
 
     int sp = 0;
     const unsigned long long all = __ballot(valid);
     if (all) {
         if (lane == 0) {
             stk_node[wave][0] = 0;
-            stk_mask[wave][0] = all;
+            stk_mask[wave][0] = make_uint2(static_cast<unsigned>(all), static_cast<unsigned>(all >> 32));
         }
         sp = 1;
     }
@@ -681,75 +732,64 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(TravArgs A) {
         --sp;
         __builtin_amdgcn_wave_barrier();
         const int node = __builtin_amdgcn_readfirstlane(stk_node[wave][sp]);
-        const unsigned long long mask_v = stk_mask[wave][sp];
-        const unsigned lo32 = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(mask_v));
-        const unsigned hi32 = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(mask_v >> 32));
+        const uint2 mv = stk_mask[wave][sp];
+        const unsigned lo32 = __builtin_amdgcn_readfirstlane(mv.x);
+        const unsigned hi32 = __builtin_amdgcn_readfirstlane(mv.y);
         const unsigned long long mask = (static_cast<unsigned long long>(hi32) << 32) | lo32;
-        const bool mine = (mask >> lane) & 1ULL;
-        float bx = 0.f, by = 0.f, bz = 0.f;
-        const int level = A.node_level[node];
+        const bool mine = (lane < 32 ? (lo32 >> lane) : (hi32 >> (lane - 32))) & 1u;
+        if (cnt > kListCap - 12) flush();
+        const int level = node_level[node];
         if (level >= kBucketLevel) {
-            // identical keys: every particle of the range exactly, skipping self
-            const int f = A.node_first[node], l = A.node_last[node];
+            // identical keys: every particle of the range, pair by pair
+            const int f = node_first[node], l = node_last[node];
             for (int q = f; q <= l; ++q) {
-                const float4 s = A.pos_sorted[q];
-                if (mine && q != t) {
-                    pair_acc<LAW>(s.x, s.y, s.z, s.w, me.x, me.y, me.z, A.soft, A.tiny, bx, by, bz);
-                    ++count;
-                }
+                const unsigned long long m = without_self(mask, q);
+                if (m) append(pos_sorted[q], m);
+                if (cnt > kListCap - 4) flush();
             }
-        } else {
-            const float4 c = A.node_com[node];
-            bool open = false;
-            if (mine) {
-                if (approximate(A.node_side[node], me.x - c.x, me.y - c.y, me.z - c.z, A.theta2, A.theta_lo)) {
-                    pair_acc<LAW>(c.x, c.y, c.z, c.w, me.x, me.y, me.z, A.soft, A.tiny, bx, by, bz);
-                    ++count;
-                } else {
-                    open = true;
-                }
-            }
-            const unsigned long long om = __ballot(open);
-            if (om) {
-                for (int slot = 0; slot < 8; ++slot) {
-                    const int ch = A.node_child[8 * node + slot];
-                    if (ch == kEmpty) continue;
-                    if (ch <= -2) {
-                        const int q = dec_leaf(ch);
-                        const float4 s = A.pos_sorted[q];
-                        if (open && q != t) {
-                            pair_acc<LAW>(s.x, s.y, s.z, s.w, me.x, me.y, me.z, A.soft, A.tiny, bx, by, bz);
-                            ++count;
-                        }
-                    } else {
-                        if (sp < kStackCap) {
-                            if (lane == 0) {
-                                stk_node[wave][sp] = ch;
-                                stk_mask[wave][sp] = om;
-                            }
-                            ++sp;
-                        } else if (lane == 0) {
-                            atomicOr(A.overflow, 1u);
-                        }
+            continue;
+        }
+        const float4 c = node_com[node];
+        const bool approx = mine && approximate(node_side[node], me.x - c.x, me.y - c.y, me.z - c.z, A.theta2,
+                                                A.theta_lo);
+        const unsigned long long am = __ballot(approx);
+        const unsigned long long om = mask & ~am;
+        if (am) append(c, am);
+        if (om) {
+            const int4 ch0 = *reinterpret_cast<const int4*>(node_child + 8 * node);
+            const int4 ch1 = *reinterpret_cast<const int4*>(node_child + 8 * node + 4);
+            const int chs[8] = {ch0.x, ch0.y, ch0.z, ch0.w, ch1.x, ch1.y, ch1.z, ch1.w};
+#pragma unroll
+            for (int slot = 0; slot < 8; ++slot) {
+                const int ch = chs[slot];
+                if (ch == kEmpty) continue;
+                if (ch <= -2) {
+                    const int q = dec_leaf(ch);
+                    const unsigned long long m = without_self(om, q);
+                    if (m) append(pos_sorted[q], m);
+                } else if (sp < kStackCap) {
+                    if (lane == 0) {
+                        stk_node[wave][sp] = ch;
+                        stk_mask[wave][sp] = make_uint2(static_cast<unsigned>(om), static_cast<unsigned>(om >> 32));
                     }
+                    ++sp;
+                } else if (lane == 0) {
+                    atomicOr(A.overflow, 1u);
                 }
             }
         }
-        kahan_add(ax, cx, bx);
-        kahan_add(ay, cy, by);
-        kahan_add(az, cz, bz);
-        __builtin_amdgcn_wave_barrier();
     }
+    if (cnt) flush();
 
     if (valid) {
-        const unsigned p = A.order[t];
-        A.acc[p] = make_float4(ax, ay, az, 0.f);
+        const unsigned p = order[t];
+        A.acc[p] = make_float4(sx, sy, sz, 0.f);
         if (A.kick) {
             float4 v = A.vel[p];
             float4 x = A.pos_in[p];
-            v.x = __builtin_fmaf(ax, A.dt, v.x);
-            v.y = __builtin_fmaf(ay, A.dt, v.y);
-            v.z = __builtin_fmaf(az, A.dt, v.z);
+            v.x = __builtin_fmaf(sx, A.dt, v.x);
+            v.y = __builtin_fmaf(sy, A.dt, v.y);
+            v.z = __builtin_fmaf(sz, A.dt, v.z);
             x.x = __builtin_fmaf(v.x, A.dts, x.x);
             x.y = __builtin_fmaf(v.y, A.dts, x.y);
             x.z = __builtin_fmaf(v.z, A.dts, x.z);
@@ -757,13 +797,13 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(TravArgs A) {
             A.pos_out[p] = x;
         }
     }
-    red[tid] = count;
+    if (lane == 0) red[wave] = count;
     __syncthreads();
-    for (int w = kTravThreads / 2; w > 0; w >>= 1) {
-        if (tid < w) red[tid] += red[tid + w];
-        __syncthreads();
+    if (tid == 0) {
+        unsigned long long tot = 0;
+        for (int w = 0; w < kTravWaves; ++w) tot += red[w];
+        atomicAdd(A.interactions, tot);
     }
-    if (tid == 0) atomicAdd(A.interactions, red[0]);
 }
 
 }  // namespace
@@ -1066,10 +1106,13 @@ nbody_status bh_step(nbody_sim* s, float dt, bool kick, hipEvent_t ev_b, hipEven
     A.dt = dt;
     A.dts = static_cast<float>(static_cast<double>(dt) / s->cfg.scale);
     if (ev_b) NB_HIP(hipEventRecord(ev_b, st));
+    const dim3 tgrid(blocks_for(n, kTravThreads));
     if (s->cfg.law == NBODY_LAW_R)
-        hipLaunchKernelGGL(traverse_kernel<NBODY_LAW_R>, dim3(blocks_for(n, kTravThreads)), dim3(kTravThreads), 0, st, A);
+        hipLaunchKernelGGL(traverse_kernel<NBODY_LAW_R>, tgrid, dim3(kTravThreads), 0, st, b->pos_sorted,
+                           b->idx[b->sorted_buf], b->com, b->side, b->level, b->child, b->first, b->last, A);
     else
-        hipLaunchKernelGGL(traverse_kernel<NBODY_LAW_P>, dim3(blocks_for(n, kTravThreads)), dim3(kTravThreads), 0, st, A);
+        hipLaunchKernelGGL(traverse_kernel<NBODY_LAW_P>, tgrid, dim3(kTravThreads), 0, st, b->pos_sorted,
+                           b->idx[b->sorted_buf], b->com, b->side, b->level, b->child, b->first, b->last, A);
     NB_HIP(hipGetLastError());
     if (ev_e) NB_HIP(hipEventRecord(ev_e, st));
     return NBODY_OK;

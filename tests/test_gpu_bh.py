@@ -108,7 +108,7 @@ def test_theta_zero_is_direct_sum(gpu):
     with _bh(pu, pu.default_config(law=pu.LAW_R, softening=soft), pos, vel, mass) as sim:
         a_d = sim.accelerations()
     assert st["interactions"] == n * (n - 1)
-    assert rel_err_rows(a_bh, a_d).max() < 2e-6
+    assert rel_err_rows(a_bh, a_d).max() < 1e-5  # two FP32 sums of the same pairs in different orders
 
 
 def test_reference_ten_steps(gpu, oracle):
@@ -138,7 +138,7 @@ def test_identical_positions_form_a_bucket(gpu):
     with _bh(pu, pu.default_config(law=pu.LAW_R, softening=soft), pos, vel, mass) as sim:
         a_d = sim.accelerations()
     assert np.all(np.isfinite(a_bh))
-    assert rel_err_rows(a_bh, a_d).max() < 2e-6
+    assert rel_err_rows(a_bh, a_d).max() < 1e-5  # two FP32 sums of the same pairs in different orders
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 17])
