@@ -30,6 +30,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "procedural-universe_amd")
 
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector peak (AMD public spec)
+# HBM-side bytes per launch of the fused direct kernel at N = 2^20 with the
+# default shape, from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a 256 MiB
+# copy (profiles/direct_force_n1m_law_r_traffic.md).
+DIRECT_1M_TRAFFIC_BYTES = 2.1854e9
 FLOP_PER_INTERACTION = 20  # Nyland, Harris & Prins (GPU Gems 3, ch. 31) convention
 METRIC = "interactions/s & % FP32 VALU peak at N=1M; batched p50/p99 step latency"
 
@@ -363,6 +367,26 @@ def bench_serve(pu, args):
     }
 
 
+def bench_copy(args):
+    """Calibration for the HBM-traffic counters: three device-to-device copies
+    of a known size through the HIP runtime's copy kernel."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = 256 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return {"workload": "copy calibration", "bytes_per_copy": nbytes, "copies": 3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -377,7 +401,7 @@ def main():
     ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--accuracy-rows", type=int, default=1024)
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU baseline sample (0 = auto)")
-    ap.add_argument("--workload", choices=["direct", "batch", "bh", "serve"], default="direct")
+    ap.add_argument("--workload", choices=["direct", "batch", "bh", "serve", "copy"], default="direct")
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--max-seconds", type=float, default=300.0)
@@ -389,6 +413,10 @@ def main():
     ap.add_argument("--galaxies", type=int, default=512)
     ap.add_argument("--baseline-steps", type=int, default=10)
     args = ap.parse_args()
+
+    if args.workload == "copy":
+        print(json.dumps(bench_copy(args)))
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -502,7 +530,12 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": DIRECT_1M_TRAFFIC_BYTES
+            if (n == 1 << 20 and timing.i_per_lane == 1 and timing.block == 256 and timing.split == 1)
+            else None,
+            "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (256 MiB copy calibration), "
+            "profiles/direct_force_n1m_law_r_traffic.md",
+            "algorithmic_bytes": 80.0 * n,
             "kernel": "direct_force_kernel (fused kick-drift)",
             "kernel_ms": force_ms,
             "flop_per_interaction": FLOP_PER_INTERACTION,

@@ -405,6 +405,23 @@ int32_t     nbody_abi_version(void);
 int32_t     nbody_device_count(void);
 
 /* ------------------------------------------------------------------------ */
+/* Snapshot files (.nbody)                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* The reference's snapshot is a raw array of nbody_particle records
+ * (SimulationState.cpp:317-331).  count: records in the file (a trailing
+ * partial record is ignored, as the reference's reader does). */
+nbody_status nbody_snapshot_count(const char* path, size_t* n);
+
+/* Read up to capacity records; recentre != 0 subtracts the centre of mass from
+ * the positions as InitParticlesFromFile does (SimulationState.cpp:252-274):
+ * long-double total mass, FP64 weighted sum, centre rounded to FP32. */
+nbody_status nbody_snapshot_read(const char* path, nbody_particle* out, size_t capacity, size_t* n,
+                                 int32_t recentre);
+
+nbody_status nbody_snapshot_write(const char* path, const nbody_particle* particles, size_t n);
+
+/* ------------------------------------------------------------------------ */
 /* Initial conditions (host, counter-based RNG: particle i's draws depend   */
 /* only on (seed, i), never on the order of generation)                     */
 /* ------------------------------------------------------------------------ */

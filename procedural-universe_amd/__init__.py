@@ -297,6 +297,11 @@ _SIGNATURES = {
     "nbody_server_poll": (ctypes.c_int, [_P, ctypes.POINTER(RequestResult), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "nbody_server_fetch": (ctypes.c_int, [_P, ctypes.c_uint64, _F32P, _F32P, ctypes.c_size_t]),
     "nbody_server_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(ServerStats)]),
+    "nbody_snapshot_count": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_size_t)]),
+    "nbody_snapshot_read": (
+        ctypes.c_int, [ctypes.c_char_p, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), ctypes.c_int32]
+    ),
+    "nbody_snapshot_write": (ctypes.c_int, [ctypes.c_char_p, _P, ctypes.c_size_t]),
     "nbody_method_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "nbody_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "nbody_last_error": (ctypes.c_char_p, []),
@@ -796,6 +801,28 @@ def batch_galaxies(count: int, seed: int, n_lo: int = 1024, n_hi: int = 65536):
             p, v, m = ic_disc_bulge(n, seed * 100003 + k, bulge_fraction=0.0)
         out.append((p, v, m))
     return out
+
+
+def read_snapshot(path: str, recentre: bool = True) -> np.ndarray:
+    """A .nbody file as PARTICLE_DTYPE records (SimulationState.cpp:229-277)."""
+    lib = load_library()
+    n = ctypes.c_size_t()
+    _check("nbody_snapshot_count", lib.nbody_snapshot_count(os.fsencode(path), ctypes.byref(n)))
+    out = np.zeros(n.value, PARTICLE_DTYPE)
+    got = ctypes.c_size_t()
+    _check(
+        "nbody_snapshot_read",
+        lib.nbody_snapshot_read(os.fsencode(path), out.ctypes.data, out.size, ctypes.byref(got), 1 if recentre else 0),
+    )
+    return out
+
+
+def write_snapshot(path: str, particles: np.ndarray):
+    """Write PARTICLE_DTYPE records as a .nbody file (SimulationState.cpp:317-331)."""
+    if particles.dtype != PARTICLE_DTYPE or not particles.flags["C_CONTIGUOUS"]:
+        raise TypeError("particles must be a C-contiguous array of PARTICLE_DTYPE")
+    _check("nbody_snapshot_write", load_library().nbody_snapshot_write(os.fsencode(path), particles.ctypes.data,
+                                                                         particles.size))
 
 
 def particles_from_soa(pos, vel, mass) -> np.ndarray:

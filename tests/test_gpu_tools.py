@@ -1,0 +1,65 @@
+"""The C++ adapter and the precompute driver on the device."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import REF_G, REF_S, ROOT, rel_err_rows
+
+pytestmark = pytest.mark.gpu
+
+INC = os.path.join(ROOT, "include")
+PKG = os.path.join(ROOT, "procedural-universe_amd")
+
+
+def _build_example(tmp_path):
+    exe = str(tmp_path / "adapter_example")
+    src = os.path.join(ROOT, "tests", "cpp", "adapter_example.cpp")
+    subprocess.run(["g++", "-std=c++17", "-I", INC, src, "-o", exe, "-L", PKG, "-lnbody", f"-Wl,-rpath,{PKG}"],
+                   check=True)
+    return exe
+
+
+@pytest.mark.parametrize("kind", [0, 2])
+def test_cpp_adapter_update_matches_oracle(gpu, oracle, tmp_path, kind):
+    pu = gpu
+    exe = _build_example(tmp_path)
+    n, steps = 512, 3
+    out = subprocess.run([exe, str(kind), str(n), str(steps)], capture_output=True, text=True, timeout=120, check=True)
+    lines = out.stdout.strip().splitlines()
+    assert lines[0] == ("Brute Force CPU" if kind == 0 else "Barnes-Hut")
+    vals = np.array([[float(x) for x in l.split()] for l in lines[1:]])
+    pos, vel, mass = pu.ic_uniform(n, 11, half=500.0, mass_lo=1e20, mass_hi=1e30)
+    dt = np.float32(0.02 / 60)
+    if kind == 0:
+        p_ref, v_ref = oracle.ref_step(pos, np.zeros((n, 3)), mass, dt, steps)
+        assert np.all(vals[:, 6] == 0)  # brute force leaves Forces zero
+    else:
+        p_ref, v_ref = oracle.bh_step(pos, np.zeros((n, 3)), mass, dt, steps, 2.0, [-4000] * 3, 8000.0)
+        assert np.any(vals[:, 6] != 0)  # Barnes-Hut leaves the last forces
+    assert rel_err_rows(vals[:, :3], p_ref).max() < 1e-4
+
+
+def test_precompute_driver_writes_snapshot(gpu, tmp_path):
+    pu = gpu
+    exe = os.path.join(PKG, "nbody_precompute")
+    out = str(tmp_path / "run.nbody")
+    r = subprocess.run([exe, "-c", "-p", "3000", "--steps", "4", "--out", out, "--method", "bh"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = pu.read_snapshot(out, recentre=False)
+    # the same run through the Python API
+    pos, vel, mass = pu.ic_uniform(3000, 0x5EED0006, half=500.0, mass_lo=1e20, mass_hi=1e30)
+    parts = pu.particles_from_soa(pos, np.zeros((3000, 3)), mass)
+    sim = pu.create_nbody_sim(pu.ENBodySim.BarnesHut)
+    sim.init(parts)
+    for _ in range(4):
+        sim.update(np.float32(0.02) * np.float32(1.0 / 60.0))
+    sim.close()
+    assert np.array_equal(got["position"], parts["position"])
+    # resume from the snapshot (-f) for two more steps
+    out2 = str(tmp_path / "run2.nbody")
+    r = subprocess.run([exe, "-f", out, "--steps", "2", "--out", out2], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert len(pu.read_snapshot(out2)) == 3000
