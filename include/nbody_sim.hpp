@@ -19,6 +19,7 @@
 #ifndef NBODY_SIM_HPP_
 #define NBODY_SIM_HPP_
 
+#include <cstddef>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -102,6 +103,15 @@ inline std::unique_ptr<INBodySim> CreateNBodySim(ENBodySim type, const nbody_con
     }
     return std::unique_ptr<INBodySim>(new DeviceNBodySim(c));
 }
+
+// The reference's factory signature (src/Sim/INBodySim.hpp:27): the Direct3D
+// device context comes first and is ignored here, so the reference's calls
+// (SimulationState.cpp:107,169,286,343) compile unchanged.
+template <class DeviceContext>
+inline std::unique_ptr<INBodySim> CreateNBodySim(DeviceContext* /*context*/, ENBodySim type) {
+    return CreateNBodySim(type);
+}
+inline std::unique_ptr<INBodySim> CreateNBodySim(std::nullptr_t, ENBodySim type) { return CreateNBodySim(type); }
 
 // src/Sim/INBodySim.cpp:29-39
 inline std::string NBodySimGetName(ENBodySim type) {

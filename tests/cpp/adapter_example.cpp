@@ -20,7 +20,7 @@ int main(int argc, char** argv) {
         particles[i].mass = mass[i];
     }
     try {
-        auto sim = CreateNBodySim(static_cast<ENBodySim>(kind));
+        auto sim = CreateNBodySim(nullptr, static_cast<ENBodySim>(kind));  // SimulationState.cpp:286
         sim->Init(particles);
         for (int s = 0; s < steps; ++s) sim->Update(0.02f / 60.0f);
     } catch (const nbody_error& e) {
