@@ -222,6 +222,11 @@ def bench_bh(pu, args):
             "tree_ms_per_step": tree_ms,
             "traversal_ms_per_step": force_ms,
             "stack_overflow": stats["stack_overflow"],
+            "walk": {
+                "list_entries_per_wave": 64.0 * stats["list_entries"] / (n * args.steps),
+                "cells_opened_per_wave": 64.0 * stats["cells_opened"] / (n * args.steps),
+                "lane_efficiency": interactions / (64.0 * max(1, stats["list_entries"])),
+            },
             "ic_seconds": t_ic,
         },
         "accuracy_vs_direct": {

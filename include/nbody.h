@@ -230,7 +230,8 @@ typedef struct nbody_bh_stats {
     uint64_t interactions;    /* cell + particle interactions since the last nbody_step began   */
     uint32_t outside_root;    /* particles clamped into a fixed root (the reference drops them) */
     uint32_t stack_overflow;  /* nonzero if a traversal stack overflowed (forces invalid)        */
-    int32_t  reserved[4];
+    uint64_t list_entries;    /* sources evaluated by a whole wavefront (64 lanes, some masked)  */
+    uint64_t cells_opened;    /* cells whose children a wavefront examined                       */
 } nbody_bh_stats;
 
 /* Counters of the Barnes-Hut steps since the last nbody_step began. */

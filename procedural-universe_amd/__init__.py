@@ -154,7 +154,8 @@ class BHStats(ctypes.Structure):
         ("interactions", ctypes.c_uint64),
         ("outside_root", ctypes.c_uint32),
         ("stack_overflow", ctypes.c_uint32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("list_entries", ctypes.c_uint64),
+        ("cells_opened", ctypes.c_uint64),
     ]
 
 
@@ -515,7 +516,8 @@ class NBodySim:
         st = BHStats()
         _check("nbody_bh_stats_get", self._lib.nbody_bh_stats_get(self._h, ctypes.byref(st)))
         return {"interactions": int(st.interactions), "outside_root": int(st.outside_root),
-                "stack_overflow": int(st.stack_overflow)}
+                "stack_overflow": int(st.stack_overflow), "list_entries": int(st.list_entries),
+                "cells_opened": int(st.cells_opened)}
 
     def bh_tree_order(self):
         """Sorted Morton keys and particle order of the last Barnes-Hut tree build."""

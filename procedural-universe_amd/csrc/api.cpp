@@ -551,9 +551,9 @@ nbody_status nbody_get_timing(nbody_sim* s, nbody_timing* out) {
     NB_HIP(hipStreamSynchronize(s->stream));
     *out = s->last;
     if (s->bh && s->n >= 2) {
-        unsigned long long it = 0;
-        NB_TRY(bh_read_counters(s->bh, s->stream, &it, nullptr, nullptr));
-        out->interactions = it;
+        BHCounters c{};
+        NB_TRY(bh_read_counters(s->bh, s->stream, &c));
+        out->interactions = c.interactions;
     }
     if (s->timed_steps > 0) {
         float ms = 0;
@@ -580,12 +580,13 @@ nbody_status nbody_bh_stats_get(nbody_sim* s, nbody_bh_stats* out) {
     NB_CHECK(s->bh, NBODY_ERR_UNSUPPORTED, "not a Barnes-Hut simulation");
     std::memset(out, 0, sizeof *out);
     NB_HIP(hipSetDevice(s->cfg.device));
-    unsigned long long it = 0;
-    unsigned outside = 0, overflow = 0;
-    NB_TRY(bh_read_counters(s->bh, s->stream, &it, &outside, &overflow));
-    out->interactions = it;
-    out->outside_root = outside;
-    out->stack_overflow = overflow;
+    BHCounters c{};
+    NB_TRY(bh_read_counters(s->bh, s->stream, &c));
+    out->interactions = c.interactions;
+    out->outside_root = c.outside;
+    out->stack_overflow = c.overflow;
+    out->list_entries = c.entries;
+    out->cells_opened = c.opened;
     return NBODY_OK;
 }
 

@@ -73,6 +73,22 @@ constexpr int kEmpty = -1;
 __host__ __device__ inline int enc_leaf(int k) { return -k - 2; }
 __host__ __device__ inline int dec_leaf(int c) { return -c - 2; }
 
+// A level-21 cell of identical keys is marked in its code; particle counts
+// stay below 2^30.
+constexpr int kBucketFlag = 1 << 30;
+
+// The eight children of an octree cell, everything the traversal needs to
+// decide about them: one 192-byte record read with three 64-byte scalar loads
+// when the cell is opened.  A child cell holds its centre of mass, G M and
+// opening threshold (below); a particle its position and G m; codes are
+// kEmpty, enc_leaf(sorted position), or the child cell's node (| kBucketFlag).
+struct alignas(64) ChildBlock {
+    float4 src[8];
+    float thr[8];
+    int code[8];
+};
+static_assert(sizeof(ChildBlock) == 192, "ChildBlock layout");
+
 struct RootParams {
     float lo[3];
     float size;    // edge length of the root cube
@@ -105,6 +121,40 @@ __device__ inline long long cell_index(float x, const RootParams& R, int a) {
     long long k = static_cast<long long>(floorf(X));
     if (X == 0.f && x < 0.f) k = -1;
     return floor_div_ll(k, R.odd) - R.base[a];
+}
+
+// ---- opening thresholds ------------------------------------------------------
+//
+// The reference approximates a cell of side d seen from p iff
+// d / |p - COM| < theta (Octree.cpp:122-124), the quotient and the length
+// (sqrtf of the squared distance r2) in FP32 and the comparison against the
+// double theta, i.e. fl(d / fl(sqrt(r2))) <= theta_lo with theta_lo the
+// largest float below theta.  Correctly rounded sqrt and division are
+// monotone, so the test is monotone in r2: false at 0 (d / 0 = inf), true at
+// +inf (d / inf = 0) when theta_lo >= 0.  The smallest float r2 for which it
+// holds is found by bisection over the ordered bit patterns of positive
+// floats, and the traversal's test becomes r2 >= thr, exactly the same
+// decision.  The side depends only on the level, so one threshold per level.
+constexpr int kThrLevels = kBucketLevel + 1;
+
+__global__ void open_threshold_kernel(const RootParams* __restrict__ R, float theta_lo, float* __restrict__ thr) {
+    const int L = threadIdx.x;
+    if (L >= kThrLevels) return;
+    const float side = ldexpf(R->size, -L);
+    auto approx = [&](unsigned bits) {
+        return __fdiv_rn(side, __fsqrt_rn(__uint_as_float(bits))) <= theta_lo;
+    };
+    const unsigned kInf = 0x7f800000u;
+    if (!approx(kInf)) {
+        thr[L] = __uint_as_float(0x7fc00000u);  // NaN: r2 >= thr never holds
+        return;
+    }
+    unsigned lo = 0u, hi = kInf;  // !approx(lo), approx(hi)
+    while (hi - lo > 1u) {
+        const unsigned mid = lo + (hi - lo) / 2u;
+        if (approx(mid)) hi = mid; else lo = mid;
+    }
+    thr[L] = __uint_as_float(hi);
 }
 
 // ---- 1. root -------------------------------------------------------------
@@ -384,43 +434,71 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(TreeArgs T) {
     if (i == 0) T.parent[0] = -1;
 }
 
-// Octree cells: kept[i] = 1 when node i starts a deeper cell than its parent.
+// Octree cells: kept[i] = 1 when node i starts a deeper cell than its parent;
+// a kept cell's child block starts with every slot empty.
 __global__ __launch_bounds__(256) void kept_kernel(const int* __restrict__ level, const int* __restrict__ parent,
-                                                   int m, int* __restrict__ kept, int* __restrict__ child) {
+                                                   int m, int* __restrict__ kept, ChildBlock* __restrict__ blocks) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= m) return;
-    kept[i] = (i == 0 || level[i] > level[parent[i]]) ? 1 : 0;
-    for (int c = 0; c < 8; ++c) child[8 * i + c] = kEmpty;
+    const int k = (i == 0 || level[i] > level[parent[i]]) ? 1 : 0;
+    kept[i] = k;
+    if (k) {
+        int4* code = reinterpret_cast<int4*>(blocks[i].code);
+        code[0] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
+        code[1] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
+    }
 }
 
-// Every kept node (but the root) and every particle outside a bucket registers
-// in the slot of its octant in the nearest kept ancestor.
-__global__ __launch_bounds__(256) void link_kernel(const unsigned long long* __restrict__ keys, int n,
-                                                   const int* __restrict__ parent,
-                                                   const int* __restrict__ leaf_parent,
-                                                   const int* __restrict__ first, const int* __restrict__ level,
-                                                   const int* __restrict__ kept, int* __restrict__ child) {
+// Every kept node (but the root) and every particle outside a bucket writes
+// itself into the slot of its octant in the nearest kept ancestor's child
+// block: (COM, G M), opening threshold and node for a cell, (position, G m)
+// for a particle.
+// Runs after the cells' centres of mass are final.
+struct LinkArgs {
+    const unsigned long long* keys;  // sorted
+    int n;
+    const int* parent;
+    const int* leaf_parent;
+    const int* first;
+    const int* level;
+    const int* kept;
+    const float4* node_com;
+    const float* thr;  // opening threshold per level
+    const float4* pos_sorted;
+    ChildBlock* blocks;
+};
+
+__global__ __launch_bounds__(256) void link_kernel(LinkArgs L) {
     const int e = blockIdx.x * 256 + threadIdx.x;  // [0, n-1): internal nodes, [n-1, 2n-1): leaves
-    const int m = n - 1;
-    if (e >= m + n) return;
+    const int m = L.n - 1;
+    if (e >= m + L.n) return;
     int q, code;
     unsigned long long key;
+    float4 src;
+    float thr;
     if (e < m) {
-        if (e == 0 || !kept[e]) return;
-        q = parent[e];
-        code = e;
-        key = keys[first[e]];
+        if (e == 0 || !L.kept[e]) return;
+        q = L.parent[e];
+        code = L.level[e] >= kBucketLevel ? (e | kBucketFlag) : e;
+        key = L.keys[L.first[e]];
+        src = L.node_com[e];
+        thr = L.thr[L.level[e]];
     } else {
         const int k = e - m;
-        q = leaf_parent[k];
+        q = L.leaf_parent[k];
         code = enc_leaf(k);
-        key = keys[k];
+        key = L.keys[k];
+        src = L.pos_sorted[k];
+        thr = 0.f;
     }
-    while (!kept[q]) q = parent[q];
-    const int L = level[q];
-    if (L >= kBucketLevel) return;  // particles of a bucket are reached by its range
-    const int slot = static_cast<int>((key >> (60 - 3 * L)) & 7ULL);
-    child[8 * q + slot] = code;
+    while (!L.kept[q]) q = L.parent[q];
+    const int lv = L.level[q];
+    if (lv >= kBucketLevel) return;  // particles of a bucket are reached by its range
+    const int slot = static_cast<int>((key >> (60 - 3 * lv)) & 7ULL);
+    ChildBlock& B = L.blocks[q];
+    B.src[slot] = src;
+    B.thr[slot] = thr;
+    B.code[slot] = code;
 }
 
 // ---- 5. FP64 mass and centre of mass ------------------------------------------------
@@ -551,35 +629,25 @@ __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
     }
 }
 
-// Node sums from the prefix differences: (M, Mx, My, Mz) in FP64.
-__global__ __launch_bounds__(256) void node_sums_kernel(const DD* __restrict__ prefix, int n,
-                                                        const int* __restrict__ first, const int* __restrict__ last,
-                                                        int m, double4* __restrict__ sums) {
+// Traversal record of every kept cell: its FP64 sums (M, Mx, My, Mz) as
+// prefix differences, then (COM, G*M) rounded once to FP32.
+__global__ __launch_bounds__(256) void cell_com_kernel(const DD* __restrict__ prefix, int n,
+                                                       const int* __restrict__ first, const int* __restrict__ last,
+                                                       const int* __restrict__ kept, int m, double G,
+                                                       float4* __restrict__ com) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= m) return;
+    if (i >= m || !kept[i]) return;
     const size_t stride = static_cast<size_t>(n) + 1;
     const int f = first[i], l = last[i] + 1;
     double r[4];
     for (int q = 0; q < 4; ++q) r[q] = dd_diff(prefix[q * stride + l], prefix[q * stride + f]);
-    sums[i] = make_double4(r[0], r[1], r[2], r[3]);
-}
-
-// Traversal records of the kept cells: (COM, G*M) in FP32 and the cell side.
-__global__ __launch_bounds__(256) void finalize_kernel(const double4* __restrict__ sums, const int* __restrict__ kept,
-                                                       const int* __restrict__ level, int m, double G,
-                                                       const RootParams* __restrict__ Rp,
-                                                       float4* __restrict__ com, float* __restrict__ side) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= m || !kept[i]) return;
-    const double4 s = sums[i];
     float cx = 0.f, cy = 0.f, cz = 0.f;
-    if (s.x > 0.0) {
-        cx = static_cast<float>(s.y / s.x);
-        cy = static_cast<float>(s.z / s.x);
-        cz = static_cast<float>(s.w / s.x);
+    if (r[0] > 0.0) {
+        cx = static_cast<float>(r[1] / r[0]);
+        cy = static_cast<float>(r[2] / r[0]);
+        cz = static_cast<float>(r[3] / r[0]);
     }
-    com[i] = make_float4(cx, cy, cz, static_cast<float>(G * s.x));
-    side[i] = ldexpf(Rp->size, -level[i]);
+    com[i] = make_float4(cx, cy, cz, static_cast<float>(G * r[0]));
 }
 
 __global__ __launch_bounds__(256) void gather_kernel(const float4* __restrict__ pos, const unsigned* __restrict__ order,
@@ -591,24 +659,18 @@ __global__ __launch_bounds__(256) void gather_kernel(const float4* __restrict__ 
 // ---- 6. traversal ---------------------------------------------------------------------
 
 struct TravArgs {
-    const float4* pos_sorted;
-    const unsigned* order;
     const float4* node_com;
-    const float* node_side;
+    const float* thr;  // opening threshold per level
     const int* node_level;
-    const int* node_child;
-    const int* node_first;
-    const int* node_last;
+    const unsigned* order;
     const float4* pos_in;   // by particle
     float4* pos_out;        // kick: by particle
     float4* vel;
     float4* acc;
-    unsigned long long* interactions;
+    unsigned long long* walk;  // [0] interactions, [1] list entries, [2] cells opened
     unsigned* overflow;
     int n;
     int kick;
-    float theta2;    // theta^2 (pre-test)
-    float theta_lo;  // largest float below theta: approximate iff side / r <= theta_lo
     float soft;
     float tiny;
     float dt;
@@ -617,26 +679,12 @@ struct TravArgs {
 
 using core::kahan_add;
 
-// The reference's opening test, Octree.cpp:122-124: approximate iff
-// side / |p - COM| < theta, with |p - COM| = sqrtf(dx*dx + dy*dy + dz*dz).  A
-// squared comparison settles all but a thin band around the threshold, where
-// the correctly rounded FP32 quotient is formed as in the reference.
-__device__ __forceinline__ bool approximate(float side, float dx, float dy, float dz, float theta2, float theta_lo) {
-    const float r2 = (dx * dx + dy * dy) + dz * dz;
-    const float lhs = side * side;
-    const float rhs = theta2 * r2;
-    if (lhs < rhs * 0.999998f) return true;
-    if (lhs > rhs * 1.000002f) return false;
-    const float r = __fsqrt_rn(r2);
-    const float q = __fdiv_rn(side, r);
-    return q <= theta_lo;
-}
-
 // The walk and the arithmetic are separated.  The walk (uniform over the
-// wave) decides, per lane and per cell, approximate or open, and appends
-// (source, lanes) entries to a list in LDS: a cell's monopole with the lanes
-// that approximate it, a single particle with the lanes that opened its
-// parent (its own lane removed).  When the list fills, every lane evaluates
+// wave) pops a cell that some lanes opened, reads its child block, and decides
+// per lane and per child: a child cell is approximated (its monopole goes to
+// the list with the lanes that approximate it) or opened (pushed with the
+// lanes that open it); a child particle goes to the list with every lane that
+// opened the parent but its own.  When the list fills, every lane evaluates
 // all entries in a tight loop - four sources at a time with grouped rsq, the
 // source's G*m zeroed for lanes outside the entry - and adds the result to its
 // compensated sum.
@@ -646,6 +694,8 @@ __device__ __forceinline__ float lane_gm(float gm, unsigned lo, unsigned hi, int
     const unsigned w = lane < 32 ? lo : hi;
     return ((w >> (lane & 31)) & 1u) ? gm : 0.f;
 }
+
+__device__ __forceinline__ bool lane_in(unsigned long long mask, int lane) { return (mask >> lane) & 1ULL; }
 
 template <int LAW>
 __device__ __forceinline__ void flush_list(const float4* lsrc, const uint2* lmask, int cnt, int lane, float px,
@@ -672,16 +722,17 @@ __device__ __forceinline__ void flush_list(const float4* lsrc, const uint2* lmas
 }
 
 template <int LAW>
-__global__ __launch_bounds__(kTravThreads) void traverse_kernel(
-    const float4* __restrict__ pos_sorted, const unsigned* __restrict__ order, const float4* __restrict__ node_com,
-    const float* __restrict__ node_side, const int* __restrict__ node_level, const int* __restrict__ node_child,
-    const int* __restrict__ node_first, const int* __restrict__ node_last, TravArgs A) {
+__global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __restrict__ pos_sorted,
+                                                                const ChildBlock* __restrict__ blocks,
+                                                                const int* __restrict__ node_first,
+                                                                const int* __restrict__ node_last, TravArgs A) {
     __shared__ int stk_node[kTravWaves][kStackCap];
     __shared__ uint2 stk_mask[kTravWaves][kStackCap];
     __shared__ float4 lst_src[kTravWaves][kListCap];
     __shared__ uint2 lst_mask[kTravWaves][kListCap];
-    __shared__ unsigned long long red[kTravWaves];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    __shared__ unsigned long long red[3][kTravWaves];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: list and stack addresses in SGPRs
     const int t = blockIdx.x * kTravThreads + tid;
     const int wave_base = blockIdx.x * kTravThreads + wave * 64;
     const bool valid = t < A.n;
@@ -689,8 +740,10 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(
     const float soft = core::in_vgpr(A.soft);
     const float tiny = core::in_vgpr(A.tiny);
     float sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
-    unsigned long long count = 0;  // wave-uniform
-    int cnt = 0;                   // list entries (wave-uniform)
+    // wave-uniform counters and list length
+    unsigned long long count = 0, entries = 0, opened = 0;
+    int cnt = 0;
+    int sp = 0;
 
     auto append = [&](const float4 src, unsigned long long mask) {
         if (lane == 0) {
@@ -698,6 +751,7 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(
             lst_mask[wave][cnt] = make_uint2(static_cast<unsigned>(mask), static_cast<unsigned>(mask >> 32));
         }
         count += static_cast<unsigned long long>(__popcll(mask));
+        ++entries;
         ++cnt;
     };
     auto flush = [&]() {
@@ -713,76 +767,94 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(
         __builtin_amdgcn_wave_barrier();
         cnt = 0;
     };
-    // the lanes of a leaf's own particle never interact with it
+    auto push = [&](int code, unsigned long long mask) {
+        if (sp < kStackCap) {
+            if (lane == 0) {
+                stk_node[wave][sp] = code;
+                stk_mask[wave][sp] = make_uint2(static_cast<unsigned>(mask), static_cast<unsigned>(mask >> 32));
+            }
+            ++sp;
+        } else if (lane == 0) {
+            atomicOr(A.overflow, 1u);
+        }
+    };
+    // A cell met by the lanes of `mask`: each approximates it (the reference's
+    // side / |p - COM| < theta as r2 >= thr, see open_threshold_kernel) or
+    // opens it.  r2 in the reference's order, (dx dx + dy dy) + dz dz, unfused.
+    auto cell = [&](const float4 c, float thr, int code, unsigned long long mask) {
+        const float dx = me.x - c.x, dy = me.y - c.y, dz = me.z - c.z;
+        const float r2 = (dx * dx + dy * dy) + dz * dz;
+        const unsigned long long am = __ballot(r2 >= thr) & mask;
+        if (am) append(c, am);
+        const unsigned long long om = mask & ~am;
+        if (om) push(code, om);
+    };
+    // the lanes of a particle's own position never interact with it
     auto without_self = [&](unsigned long long m, int q) {
         const int l = q - wave_base;
         return (l >= 0 && l < 64) ? (m & ~(1ULL << l)) : m;
     };
 
-    int sp = 0;
     const unsigned long long all = __ballot(valid);
     if (all) {
-        if (lane == 0) {
-            stk_node[wave][0] = 0;
-            stk_mask[wave][0] = make_uint2(static_cast<unsigned>(all), static_cast<unsigned>(all >> 32));
-        }
-        sp = 1;
+        const int root_level = A.node_level[0];
+        cell(A.node_com[0], A.thr[root_level], root_level >= kBucketLevel ? kBucketFlag : 0, all);
     }
     while (sp > 0) {
         --sp;
         __builtin_amdgcn_wave_barrier();
-        const int node = __builtin_amdgcn_readfirstlane(stk_node[wave][sp]);
+        const int code = __builtin_amdgcn_readfirstlane(stk_node[wave][sp]);
         const uint2 mv = stk_mask[wave][sp];
         const unsigned lo32 = __builtin_amdgcn_readfirstlane(mv.x);
         const unsigned hi32 = __builtin_amdgcn_readfirstlane(mv.y);
         const unsigned long long mask = (static_cast<unsigned long long>(hi32) << 32) | lo32;
-        const bool mine = (lane < 32 ? (lo32 >> lane) : (hi32 >> (lane - 32))) & 1u;
-        if (cnt > kListCap - 12) flush();
-        const int level = node_level[node];
-        if (level >= kBucketLevel) {
+        ++opened;
+        if (code & kBucketFlag) {
             // identical keys: every particle of the range, pair by pair
+            const int node = code & ~kBucketFlag;
             const int f = node_first[node], l = node_last[node];
             for (int q = f; q <= l; ++q) {
+                if (cnt > kListCap - 4) flush();
                 const unsigned long long m = without_self(mask, q);
                 if (m) append(pos_sorted[q], m);
-                if (cnt > kListCap - 4) flush();
             }
             continue;
         }
-        const float4 c = node_com[node];
-        const bool approx = mine && approximate(node_side[node], me.x - c.x, me.y - c.y, me.z - c.z, A.theta2,
-                                                A.theta_lo);
-        const unsigned long long am = __ballot(approx);
-        const unsigned long long om = mask & ~am;
-        if (am) append(c, am);
-        if (om) {
-            const int4 ch0 = *reinterpret_cast<const int4*>(node_child + 8 * node);
-            const int4 ch1 = *reinterpret_cast<const int4*>(node_child + 8 * node + 4);
-            const int chs[8] = {ch0.x, ch0.y, ch0.z, ch0.w, ch1.x, ch1.y, ch1.z, ch1.w};
+        if (cnt > kListCap - 8) flush();
+        // The whole block in scalar registers at once (the empty asm keeps the
+        // compiler from sinking each slot's loads into that slot's branch,
+        // which would wait on memory once per cache line).
+        const ChildBlock* B = blocks + code;
+        const int4 c0 = *reinterpret_cast<const int4*>(B->code);
+        const int4 c1 = *reinterpret_cast<const int4*>(B->code + 4);
+        int codes[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        float4 srcs[8];
+        float thrs[8];
 #pragma unroll
-            for (int slot = 0; slot < 8; ++slot) {
-                const int ch = chs[slot];
-                if (ch == kEmpty) continue;
-                if (ch <= -2) {
-                    const int q = dec_leaf(ch);
-                    const unsigned long long m = without_self(om, q);
-                    if (m) append(pos_sorted[q], m);
-                } else if (sp < kStackCap) {
-                    if (lane == 0) {
-                        stk_node[wave][sp] = ch;
-                        stk_mask[wave][sp] = make_uint2(static_cast<unsigned>(om), static_cast<unsigned>(om >> 32));
-                    }
-                    ++sp;
-                } else if (lane == 0) {
-                    atomicOr(A.overflow, 1u);
-                }
+        for (int slot = 0; slot < 8; ++slot) {
+            srcs[slot] = B->src[slot];
+            thrs[slot] = B->thr[slot];
+        }
+#pragma unroll
+        for (int slot = 0; slot < 8; ++slot)
+            asm volatile("" : "+s"(srcs[slot].x), "+s"(srcs[slot].y), "+s"(srcs[slot].z), "+s"(srcs[slot].w),
+                         "+s"(thrs[slot]), "+s"(codes[slot]));
+#pragma unroll
+        for (int slot = 0; slot < 8; ++slot) {
+            const int c = codes[slot];
+            if (c == kEmpty) continue;
+            if (c <= -2) {
+                const unsigned long long m = without_self(mask, dec_leaf(c));
+                if (m) append(srcs[slot], m);
+            } else {
+                cell(srcs[slot], thrs[slot], c, mask);
             }
         }
     }
     if (cnt) flush();
 
     if (valid) {
-        const unsigned p = order[t];
+        const unsigned p = A.order[t];
         A.acc[p] = make_float4(sx, sy, sz, 0.f);
         if (A.kick) {
             float4 v = A.vel[p];
@@ -797,12 +869,16 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(
             A.pos_out[p] = x;
         }
     }
-    if (lane == 0) red[wave] = count;
+    if (lane == 0) {
+        red[0][wave] = count;
+        red[1][wave] = entries;
+        red[2][wave] = opened;
+    }
     __syncthreads();
-    if (tid == 0) {
+    if (tid < 3) {
         unsigned long long tot = 0;
-        for (int w = 0; w < kTravWaves; ++w) tot += red[w];
-        atomicAdd(A.interactions, tot);
+        for (int w = 0; w < kTravWaves; ++w) tot += red[tid][w];
+        atomicAdd(A.walk + tid, tot);
     }
 }
 
@@ -827,16 +903,15 @@ struct BHState {
     int* last = nullptr;
     int* level = nullptr;
     int* kept = nullptr;
-    int* child = nullptr;
+    ChildBlock* blocks = nullptr;
     unsigned* chunk_sums = nullptr;  // sort scan
     size_t chunk_elems = 0;
     DD* prefix = nullptr;            // [4][n + 1] double-double prefix sums
     DD* mass_chunk = nullptr;        // [4][nchunks]
-    double4* sums = nullptr;
     float4* com = nullptr;
-    float* side = nullptr;
+    float* thr = nullptr;                    // [kThrLevels] opening thresholds
     unsigned* counters = nullptr;            // [0] outside, [1] overflow
-    unsigned long long* interactions = nullptr;
+    unsigned long long* walk = nullptr;      // [3] traversal counters
     int sorted_buf = 0;                      // which keys/idx buffer holds the sorted result
     RootParams fixed{};
     bool fixed_root = false;
@@ -868,13 +943,12 @@ void bh_release(BHState* b) {
     bh_free(b->last);
     bh_free(b->level);
     bh_free(b->kept);
-    bh_free(b->child);
+    bh_free(b->blocks);
     bh_free(b->chunk_sums);
     bh_free(b->prefix);
     bh_free(b->mass_chunk);
-    bh_free(b->sums);
     bh_free(b->com);
-    bh_free(b->side);
+    bh_free(b->thr);
     b->cap = 0;
     b->hist_elems = 0;
 }
@@ -939,7 +1013,7 @@ nbody_status bh_create(nbody_sim* s) {
     }
     hipError_t e1 = hipMalloc(reinterpret_cast<void**>(&b->root), sizeof(RootParams));
     hipError_t e2 = hipMalloc(reinterpret_cast<void**>(&b->counters), 2 * sizeof(unsigned));
-    hipError_t e3 = hipMalloc(reinterpret_cast<void**>(&b->interactions), sizeof(unsigned long long));
+    hipError_t e3 = hipMalloc(reinterpret_cast<void**>(&b->walk), 3 * sizeof(unsigned long long));
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
         bh_destroy(b);
         set_error("hipMalloc failed for Barnes-Hut control blocks");
@@ -954,7 +1028,7 @@ nbody_status bh_create(nbody_sim* s) {
         }
     }
     (void)hipMemset(b->counters, 0, 2 * sizeof(unsigned));
-    (void)hipMemset(b->interactions, 0, sizeof(unsigned long long));
+    (void)hipMemset(b->walk, 0, 3 * sizeof(unsigned long long));
     s->bh = b;
     return NBODY_OK;
 }
@@ -964,7 +1038,7 @@ void bh_destroy(BHState* b) {
     bh_release(b);
     bh_free(b->root);
     bh_free(b->counters);
-    bh_free(b->interactions);
+    bh_free(b->walk);
     delete b;
 }
 
@@ -989,13 +1063,12 @@ nbody_status bh_resize(BHState* b, size_t n, nbody_sim*) {
     NB_TRY(bh_alloc(b->last, m));
     NB_TRY(bh_alloc(b->level, m));
     NB_TRY(bh_alloc(b->kept, m));
-    NB_TRY(bh_alloc(b->child, 8 * m));
+    NB_TRY(bh_alloc(b->blocks, m));
     NB_TRY(bh_alloc(b->chunk_sums, static_cast<size_t>(blocks_for(256 * nb, kScanChunk))));
     NB_TRY(bh_alloc(b->prefix, 4 * (n + 1)));
     NB_TRY(bh_alloc(b->mass_chunk, 4 * static_cast<size_t>(blocks_for(n + 1, kMassChunk))));
-    NB_TRY(bh_alloc(b->sums, m));
     NB_TRY(bh_alloc(b->com, m));
-    NB_TRY(bh_alloc(b->side, m));
+    NB_TRY(bh_alloc(b->thr, kThrLevels));
     b->cap = n;
     return NBODY_OK;
 }
@@ -1032,18 +1105,18 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     TreeArgs T{keys, n, b->left, b->right, b->parent, b->leaf_parent, b->first, b->last, b->level};
     hipLaunchKernelGGL(radix_tree_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, T);
     hipLaunchKernelGGL(kept_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->level, b->parent, m, b->kept,
-                       b->child);
-    hipLaunchKernelGGL(link_kernel, dim3(blocks_for(m + n, 256)), dim3(256), 0, st, keys, n, b->parent,
-                       b->leaf_parent, b->first, b->level, b->kept, b->child);
+                       b->blocks);
     const int mchunks = blocks_for(n + 1, kMassChunk);
     MassArgs M{b->pos_sorted, order, s->mass_dev, n, b->prefix, b->mass_chunk, mchunks};
     hipLaunchKernelGGL(mass_reduce_kernel, dim3(mchunks), dim3(kMassThreads), 0, st, M);
     hipLaunchKernelGGL(mass_chunks_kernel, dim3(1), dim3(kMassThreads), 0, st, M);
     hipLaunchKernelGGL(mass_prefix_kernel, dim3(mchunks), dim3(kMassThreads), 0, st, M);
-    hipLaunchKernelGGL(node_sums_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->prefix, n, b->first, b->last, m,
-                       b->sums);
-    hipLaunchKernelGGL(finalize_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->sums, b->kept, b->level, m,
-                       s->cfg.G, b->root, b->com, b->side);
+    hipLaunchKernelGGL(cell_com_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->prefix, n, b->first, b->last,
+                       b->kept, m, s->cfg.G, b->com);
+    hipLaunchKernelGGL(open_threshold_kernel, dim3(1), dim3(64), 0, st, b->root, theta_below(s->cfg.theta), b->thr);
+    LinkArgs L{keys, n, b->parent, b->leaf_parent, b->first, b->level, b->kept, b->com, b->thr, b->pos_sorted,
+               b->blocks};
+    hipLaunchKernelGGL(link_kernel, dim3(blocks_for(m + n, 256)), dim3(256), 0, st, L);
     NB_HIP(hipGetLastError());
     return NBODY_OK;
 }
@@ -1083,24 +1156,18 @@ nbody_status bh_step(nbody_sim* s, float dt, bool kick, hipEvent_t ev_b, hipEven
     if (te) NB_HIP(hipEventRecord(te, st));
 
     TravArgs A{};
-    A.pos_sorted = b->pos_sorted;
-    A.order = b->idx[b->sorted_buf];
     A.node_com = b->com;
-    A.node_side = b->side;
+    A.thr = b->thr;
     A.node_level = b->level;
-    A.node_child = b->child;
-    A.node_first = b->first;
-    A.node_last = b->last;
+    A.order = b->idx[b->sorted_buf];
     A.pos_in = s->pos[s->cur];
     A.pos_out = s->pos[s->cur ^ 1];
     A.vel = s->vel;
     A.acc = s->acc;
-    A.interactions = b->interactions;
+    A.walk = b->walk;
     A.overflow = b->counters + 1;
     A.n = n;
     A.kick = kick ? 1 : 0;
-    A.theta2 = static_cast<float>(s->cfg.theta * s->cfg.theta);
-    A.theta_lo = theta_below(s->cfg.theta);
     A.soft = static_cast<float>(s->cfg.softening);
     A.tiny = direct_tiny_for(s->max_gm);
     A.dt = dt;
@@ -1108,11 +1175,11 @@ nbody_status bh_step(nbody_sim* s, float dt, bool kick, hipEvent_t ev_b, hipEven
     if (ev_b) NB_HIP(hipEventRecord(ev_b, st));
     const dim3 tgrid(blocks_for(n, kTravThreads));
     if (s->cfg.law == NBODY_LAW_R)
-        hipLaunchKernelGGL(traverse_kernel<NBODY_LAW_R>, tgrid, dim3(kTravThreads), 0, st, b->pos_sorted,
-                           b->idx[b->sorted_buf], b->com, b->side, b->level, b->child, b->first, b->last, A);
+        hipLaunchKernelGGL(traverse_kernel<NBODY_LAW_R>, tgrid, dim3(kTravThreads), 0, st, b->pos_sorted, b->blocks,
+                           b->first, b->last, A);
     else
-        hipLaunchKernelGGL(traverse_kernel<NBODY_LAW_P>, tgrid, dim3(kTravThreads), 0, st, b->pos_sorted,
-                           b->idx[b->sorted_buf], b->com, b->side, b->level, b->child, b->first, b->last, A);
+        hipLaunchKernelGGL(traverse_kernel<NBODY_LAW_P>, tgrid, dim3(kTravThreads), 0, st, b->pos_sorted, b->blocks,
+                           b->first, b->last, A);
     NB_HIP(hipGetLastError());
     if (ev_e) NB_HIP(hipEventRecord(ev_e, st));
     return NBODY_OK;
@@ -1120,20 +1187,21 @@ nbody_status bh_step(nbody_sim* s, float dt, bool kick, hipEvent_t ev_b, hipEven
 
 nbody_status bh_reset_counters(BHState* b, hipStream_t st) {
     NB_HIP(hipMemsetAsync(b->counters, 0, 2 * sizeof(unsigned), st));
-    NB_HIP(hipMemsetAsync(b->interactions, 0, sizeof(unsigned long long), st));
+    NB_HIP(hipMemsetAsync(b->walk, 0, 3 * sizeof(unsigned long long), st));
     return NBODY_OK;
 }
 
-nbody_status bh_read_counters(BHState* b, hipStream_t st, unsigned long long* interactions, unsigned* outside,
-                              unsigned* overflow) {
+nbody_status bh_read_counters(BHState* b, hipStream_t st, BHCounters* out) {
     unsigned c[2] = {0, 0};
-    unsigned long long it = 0;
+    unsigned long long w[3] = {0, 0, 0};
     NB_HIP(hipMemcpyAsync(c, b->counters, sizeof c, hipMemcpyDeviceToHost, st));
-    NB_HIP(hipMemcpyAsync(&it, b->interactions, sizeof it, hipMemcpyDeviceToHost, st));
+    NB_HIP(hipMemcpyAsync(w, b->walk, sizeof w, hipMemcpyDeviceToHost, st));
     NB_HIP(hipStreamSynchronize(st));
-    if (interactions) *interactions = it;
-    if (outside) *outside = c[0];
-    if (overflow) *overflow = c[1];
+    out->interactions = w[0];
+    out->entries = w[1];
+    out->opened = w[2];
+    out->outside = c[0];
+    out->overflow = c[1];
     return NBODY_OK;
 }
 

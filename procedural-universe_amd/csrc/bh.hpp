@@ -23,10 +23,16 @@ nbody_status bh_resize(BHState* b, size_t n, nbody_sim* s);
 nbody_status bh_step(nbody_sim* s, float dt, bool kick, hipEvent_t ev_force_begin, hipEvent_t ev_force_end);
 // Check a fixed root (corner on the root's level-21 grid); out_params may be null.
 nbody_status bh_fixed_root(const nbody_config* c, void* out_params);
-// Interaction / clamp / overflow counters of the steps since the last reset.
+// Counters of the steps since the last reset.
+struct BHCounters {
+    unsigned long long interactions;  // lanes x list entries they take part in
+    unsigned long long entries;       // list entries (sources evaluated by a whole wave)
+    unsigned long long opened;        // cells opened (stack pops)
+    unsigned outside;                 // particles clamped into a fixed root
+    unsigned overflow;                // nonzero if a traversal stack overflowed
+};
 nbody_status bh_reset_counters(BHState* b, hipStream_t st);
-nbody_status bh_read_counters(BHState* b, hipStream_t st, unsigned long long* interactions, unsigned* outside,
-                              unsigned* overflow);
+nbody_status bh_read_counters(BHState* b, hipStream_t st, BHCounters* out);
 // Sorted keys and particle order of the last tree build.
 nbody_status bh_export_order(nbody_sim* s, uint64_t* keys, uint32_t* order, size_t n);
 

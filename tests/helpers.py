@@ -196,3 +196,26 @@ def rel_err_rows(a, ref):
     num = np.linalg.norm(a - ref, axis=1)
     den = np.linalg.norm(ref, axis=1)
     return num / np.where(den > 0, den, 1.0)
+
+
+def opening_boundary_case(k):
+    """Three particles under the reference's fixed root [-4000, 4000)^3: an
+    equal-mass pair whose deepest common cell is [0, 125)^3 (level 6, side
+    125, centre of mass (62.5, 62.5, 62.5) exactly), and a target on the x axis
+    through that centre, k FP32 ulps beyond distance 250 = 2 * side.  With
+    theta = 0.5 the reference's side / |p - COM| < theta holds from k = 1 on:
+    at k = 0 the quotient is exactly 0.5 and the cell is opened.
+    Returns (pos, mass, index of the target)."""
+    x = np.float32(312.5)
+    for _ in range(abs(k)):
+        x = np.nextafter(x, np.float32(np.inf if k > 0 else -np.inf), dtype=np.float32)
+    pos = np.array([[31.25, 31.25, 31.25], [93.75, 93.75, 93.75], [x, 62.5, 62.5]], np.float32)
+    mass = np.array([1e25, 1e25, 1e20])
+    return pos, mass, 2
+
+
+def law_r_force(p, q, mp, mq, G=REF_G, S=REF_S):
+    """FP64 law-R force on p from q: G mp mq / (r^2 + S) along (q - p)."""
+    d = np.asarray(q, np.float64) - np.asarray(p, np.float64)
+    r = np.linalg.norm(d)
+    return G * mp * mq / (r * r + S) * d / r

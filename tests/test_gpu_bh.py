@@ -10,7 +10,7 @@ Octree / BarnesHut (src/Sim/Octree.cpp:16-145, src/Sim/BarnesHut.cpp:44-110).
 import numpy as np
 import pytest
 
-from helpers import REF_G, REF_S, rel_err_rows
+from helpers import REF_G, REF_S, opening_boundary_case, rel_err_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -200,3 +200,18 @@ def test_two_disc_accuracy_vs_direct(gpu, oracle):
     assert np.sqrt(np.mean(rerr ** 2)) < 1e-4
     assert rms < 3e-2
     assert st["interactions"] < n * n / 4
+
+
+@pytest.mark.parametrize("k", [-2, -1, 0, 1, 2, 3])
+def test_opening_decision_at_the_boundary(gpu, oracle, k):
+    """The device's per-level opening threshold makes the reference's decision
+    exactly where side / |p - COM| is 0.5 or one rounding step below it."""
+    pu = gpu
+    pos, mass, t = opening_boundary_case(k)
+    vel = np.zeros_like(pos)
+    cfg = pu.reference_config(pu.METHOD_BARNES_HUT, theta=0.5)
+    with _bh(pu, cfg, pos, vel, mass) as sim:
+        a = sim.accelerations()
+    F, _ = oracle.bh_forces(pos, mass, 0.5, REF_LO, REF_SIZE)
+    err = rel_err_rows(a, F / mass[:, None])
+    assert err.max() < 1e-5, err

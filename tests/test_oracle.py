@@ -11,7 +11,7 @@ from fractions import Fraction
 import numpy as np
 import pytest
 
-from helpers import REF_G, REF_S, REF_SCALE, rel_err_rows
+from helpers import REF_G, REF_S, REF_SCALE, law_r_force, opening_boundary_case, rel_err_rows
 
 
 def _two_body(r=3.0, m0=2.0e24, m1=5.0e25):
@@ -326,3 +326,19 @@ def test_bh_drops_particles_outside_root(oracle):
     mass = np.array([1e25, 1e25, 1e25])
     _, st = oracle.bh_forces(pos, mass, 0.5, [-4000] * 3, 8000.0)
     assert st["dropped"] == 1
+
+
+@pytest.mark.parametrize("k", [-2, -1, 0, 1, 2, 3])
+def test_bh_opening_test_at_the_boundary(oracle, k):
+    """Octree.cpp:122-124 approximates iff side / r < theta.  A two-particle
+    cell of side 125 seen from exactly 250 (quotient 0.5, theta 0.5) is opened;
+    one FP32 ulp farther it is approximated by its monopole.  The pair's exact
+    forces and the monopole differ by about 5%, so the decision is visible."""
+    pos, mass, t = opening_boundary_case(k)
+    F, st = oracle.bh_forces(pos, mass, 0.5, [-4000] * 3, 8000.0)
+    assert st["dropped"] == 0
+    exact = law_r_force(pos[t], pos[0], mass[t], mass[0]) + law_r_force(pos[t], pos[1], mass[t], mass[1])
+    mono = law_r_force(pos[t], [62.5, 62.5, 62.5], mass[t], mass[0] + mass[1])
+    assert np.linalg.norm(exact - mono) > 1e-2 * np.linalg.norm(exact)
+    want = mono if k >= 1 else exact
+    assert np.linalg.norm(F[t] - want) < 1e-5 * np.linalg.norm(want)
