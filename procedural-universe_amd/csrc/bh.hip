@@ -555,77 +555,112 @@ __device__ inline void leaf_vals(const MassArgs& A, int k, double v[4]) {
     v[3] = m * static_cast<double>(p.z);
 }
 
-// Block-wide inclusive scan of one DD per thread (Hillis-Steele in LDS).
-__device__ inline DD block_scan_dd(DD x, DD* sh) {
-    const int t = threadIdx.x;
-    sh[t] = x;
-    __syncthreads();
-    for (int off = 1; off < kMassThreads; off <<= 1) {
-        const DD a = t >= off ? sh[t - off] : DD{0.0, 0.0};
-        __syncthreads();
-        sh[t] = dd_add(a, sh[t]);
-        __syncthreads();
-    }
-    const DD r = sh[t];
-    __syncthreads();
-    return r;
+__device__ inline DD shfl_up_dd(DD x, int off) {
+    return DD{__shfl_up(x.hi, off, 64), __shfl_up(x.lo, off, 64)};
 }
 
-// Sum of each chunk, per quantity.
-__global__ __launch_bounds__(kMassThreads) void mass_reduce_kernel(MassArgs A) {
-    __shared__ DD sh[kMassThreads];
-    const int base = blockIdx.x * kMassChunk + threadIdx.x * kMassPer;
-    DD acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    for (int k = 0; k < kMassPer; ++k) {
-        double v[4];
-        leaf_vals(A, base + k, v);
-        for (int q = 0; q < 4; ++q) acc[q] = dd_add(acc[q], DD{v[q], 0.0});
+// Inclusive scan over the 64 lanes of a wave (fixed order: earlier lanes on
+// the left of every addition).
+__device__ inline DD wave_scan_dd(DD x, int lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const DD y = shfl_up_dd(x, off);
+        if (lane >= off) x = dd_add(y, x);
     }
-    for (int q = 0; q < 4; ++q) {
-        const DD incl = block_scan_dd(acc[q], sh);
-        if (threadIdx.x == kMassThreads - 1) A.chunk[q * A.nchunks + blockIdx.x] = incl;
+    return x;
+}
+
+// The exclusive value of a lane from the inclusive scan: the previous lane's.
+__device__ inline DD wave_exclusive(DD incl, int lane) {
+    const DD e = shfl_up_dd(incl, 1);
+    return lane > 0 ? e : DD{0.0, 0.0};
+}
+
+constexpr int kMassWaves = kMassThreads / 64;
+
+// Chunk c covers elements [c * kMassChunk, (c + 1) * kMassChunk) of [0, n]
+// (element n is 0: its exclusive prefix is the total) in kMassPer rows of
+// kMassThreads consecutive elements, read with coalesced loads.  A row is
+// scanned by wave scans plus the totals of the waves before it.
+
+// Sum of each chunk, per quantity (rows in order, a row's waves in order).
+__global__ __launch_bounds__(kMassThreads) void mass_reduce_kernel(MassArgs A) {
+    __shared__ DD wsum[4][kMassWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    DD acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int r = 0; r < kMassPer; ++r) {
+        const int i = blockIdx.x * kMassChunk + r * kMassThreads + threadIdx.x;
+        double v[4];
+        leaf_vals(A, i, v);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const DD incl = wave_scan_dd(DD{v[q], 0.0}, lane);
+            const DD tot{__shfl(incl.hi, 63, 64), __shfl(incl.lo, 63, 64)};
+            acc[q] = dd_add(acc[q], tot);
+        }
+    }
+    if (lane == 0)
+        for (int q = 0; q < 4; ++q) wsum[q][wave] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        DD t{0.0, 0.0};
+        for (int w = 0; w < kMassWaves; ++w) t = dd_add(t, wsum[threadIdx.x][w]);
+        A.chunk[threadIdx.x * A.nchunks + blockIdx.x] = t;
     }
 }
 
 // Exclusive scan of the chunk sums (one workgroup, quantities in turn).
 __global__ __launch_bounds__(kMassThreads) void mass_chunks_kernel(MassArgs A) {
-    __shared__ DD sh[kMassThreads];
+    __shared__ DD wsum[kMassWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int q = 0; q < 4; ++q) {
         DD carry{0.0, 0.0};
         for (int b = 0; b < A.nchunks; b += kMassThreads) {
             const int i = b + threadIdx.x;
             const DD v = i < A.nchunks ? A.chunk[q * A.nchunks + i] : DD{0.0, 0.0};
-            const DD incl = block_scan_dd(v, sh);
-            const DD ex = dd_add(incl, DD{-v.hi, -v.lo});
-            const DD tot = sh[kMassThreads - 1];
+            const DD incl = wave_scan_dd(v, lane);
+            if (lane == 63) wsum[wave] = incl;
             __syncthreads();
-            if (i < A.nchunks) A.chunk[q * A.nchunks + i] = dd_add(carry, ex);
-            carry = dd_add(carry, tot);
+            DD before = carry;
+            for (int w = 0; w < wave; ++w) before = dd_add(before, wsum[w]);
+            DD row = carry;
+            for (int w = 0; w < kMassWaves; ++w) row = dd_add(row, wsum[w]);
+            __syncthreads();
+            const DD excl = dd_add(before, wave_exclusive(incl, lane));
+            if (i < A.nchunks) A.chunk[q * A.nchunks + i] = excl;
+            carry = row;
         }
     }
 }
 
-// Exclusive prefix P[k] of every particle k, and P[n] (the total).
+// Exclusive prefix P[k] of every element k in [0, n]; coalesced rows.
 __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
-    __shared__ DD sh[kMassThreads];
-    const int base = blockIdx.x * kMassChunk + threadIdx.x * kMassPer;
+    __shared__ DD wsum[4][kMassWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t stride = static_cast<size_t>(A.n) + 1;
-    for (int q = 0; q < 4; ++q) {
-        DD run{0.0, 0.0};
-        for (int k = 0; k < kMassPer; ++k) {
-            double v[4];
-            leaf_vals(A, base + k, v);
-            run = dd_add(run, DD{v[q], 0.0});
+    DD carry[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) carry[q] = A.chunk[q * A.nchunks + blockIdx.x];
+    for (int r = 0; r < kMassPer; ++r) {
+        const int i = blockIdx.x * kMassChunk + r * kMassThreads + threadIdx.x;
+        double v[4];
+        leaf_vals(A, i, v);
+        DD incl[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            incl[q] = wave_scan_dd(DD{v[q], 0.0}, lane);
+            if (lane == 63) wsum[q][wave] = incl[q];
         }
-        const DD incl = block_scan_dd(run, sh);
-        DD p = dd_add(A.chunk[q * A.nchunks + blockIdx.x], dd_add(incl, DD{-run.hi, -run.lo}));
-        for (int k = 0; k < kMassPer; ++k) {
-            const int i = base + k;
-            if (i <= A.n) A.prefix[q * stride + i] = p;
-            double v[4];
-            leaf_vals(A, i, v);
-            p = dd_add(p, DD{v[q], 0.0});
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            DD before = carry[q];
+            for (int w = 0; w < wave; ++w) before = dd_add(before, wsum[q][w]);
+            const DD ex = wave_exclusive(incl[q], lane);
+            if (i <= A.n) A.prefix[q * stride + i] = dd_add(before, ex);
+            for (int w = 0; w < kMassWaves; ++w) carry[q] = dd_add(carry[q], wsum[q][w]);
         }
+        __syncthreads();
     }
 }
 
@@ -690,12 +725,12 @@ using core::kahan_add;
 // compensated sum.
 constexpr int kListCap = 128;
 
+// G*m where the lane's bit of the entry mask is set, else 0 (the bit
+// sign-extended to a full-width and-mask).
 __device__ __forceinline__ float lane_gm(float gm, unsigned lo, unsigned hi, int lane) {
     const unsigned w = lane < 32 ? lo : hi;
-    return ((w >> (lane & 31)) & 1u) ? gm : 0.f;
+    return __int_as_float(__float_as_int(gm) & __builtin_amdgcn_sbfe(static_cast<int>(w), lane & 31, 1));
 }
-
-__device__ __forceinline__ bool lane_in(unsigned long long mask, int lane) { return (mask >> lane) & 1ULL; }
 
 template <int LAW>
 __device__ __forceinline__ void flush_list(const float4* lsrc, const uint2* lmask, int cnt, int lane, float px,
