@@ -286,7 +286,7 @@ def bench_serve(pu, args):
     steps = rng.integers(10, 201, m)
     cfg = pu.server_config(law=pu.LAW_R, softening=args.eps ** 2, theta=0.6, bh_threshold=32768,
                            memory_budget=int(args.memory_gib * (1 << 30)), direct_capacity=64 << 20,
-                           max_active_bh=args.max_bh, dt=args.dt)
+                           max_active_bh=args.max_bh, dt=args.dt, bh_steps_ahead=args.bh_ahead)
     srv = pu.Server(cfg)
     offered_direct = float(sum(int(n) * int(n) * int(s) for n, s in zip(sizes, steps) if n < 32768))
     t0 = srv.clock()
@@ -349,6 +349,7 @@ def bench_serve(pu, args):
             f"Barnes-Hut theta=0.6, eps={args.eps}, dt={args.dt}",
             "memory_budget_gib": args.memory_gib,
             "max_active_bh": args.max_bh,
+            "bh_steps_ahead": args.bh_ahead,
         },
         "latency_s": {
             "p50": pct(lat, 50), "p99": pct(lat, 99),
@@ -411,7 +412,8 @@ def main():
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--max-seconds", type=float, default=300.0)
     ap.add_argument("--memory-gib", type=float, default=96.0)
-    ap.add_argument("--max-bh", type=int, default=64)
+    ap.add_argument("--max-bh", type=int, default=16, help="serve: concurrent Barnes-Hut requests")
+    ap.add_argument("--bh-ahead", type=int, default=1, help="serve: Barnes-Hut steps enqueued per request at a time")
     ap.add_argument("--progress", default="", help="append serve progress lines to this file")
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--subsample", type=int, default=65536)

@@ -155,6 +155,11 @@ nbody_status nbody_step(nbody_sim* sim, float dt, int32_t nsteps);
 /* Wait for every step enqueued on the handle. */
 nbody_status nbody_synchronize(nbody_sim* sim);
 
+/* Without waiting: *done = 1 when every step enqueued on the handle has
+ * completed, else 0.  No counterpart in the reference (its Update is
+ * synchronous); for callers that overlap several handles. */
+nbody_status nbody_query(nbody_sim* sim, int32_t* done);
+
 #define NBODY_FIELD_POSITION 1u
 #define NBODY_FIELD_VELOCITY 2u
 #define NBODY_FIELD_FORCES   4u
@@ -215,6 +220,13 @@ typedef struct nbody_timing {
     int32_t split;
     int32_t reserved[5];
 } nbody_timing;
+
+/* Replay nbody_step from hipGraphs: each distinct (nsteps, dt) sequence is
+ * captured on first use and then launched as one graph (a Barnes-Hut step is
+ * about forty kernels).  Results are bit-identical to plain launches.  Not
+ * used while timing is on; a new initial state drops the graphs.  No
+ * counterpart in the reference; for callers that step many small handles. */
+nbody_status nbody_set_graph(nbody_sim* sim, int32_t enable);
 
 /* Turn per-kernel HIP-event timing of nbody_step on or off (default off). */
 nbody_status nbody_set_timing(nbody_sim* sim, int32_t enable);
@@ -299,6 +311,8 @@ nbody_status nbody_batch_step(nbody_batch* batch, float dt, int32_t nsteps);
 nbody_status nbody_batch_step_sequential(nbody_batch* batch, float dt, int32_t nsteps, int32_t split_small);
 
 nbody_status nbody_batch_synchronize(nbody_batch* batch);
+/* Without waiting: *done = 1 when every step enqueued on the batch has completed. */
+nbody_status nbody_batch_query(nbody_batch* batch, int32_t* done);
 nbody_status nbody_batch_read(nbody_batch* batch, int32_t id, float* pos, float* vel, float* acc, size_t n);
 nbody_status nbody_batch_get_info(nbody_batch* batch, nbody_batch_info* out);
 nbody_status nbody_batch_set_tuning(nbody_batch* batch, const nbody_batch_tuning* tuning);
@@ -336,7 +350,8 @@ typedef struct nbody_server_config {
     int32_t  max_active_bh;    /* concurrent Barnes-Hut requests                            */
     float    dt;               /* step of every request                                    */
     int32_t  keep_results;     /* keep final states for nbody_server_fetch                 */
-    int32_t  reserved[8];
+    int32_t  bh_steps_ahead;   /* Barnes-Hut steps enqueued per request between checks (0: 1) */
+    int32_t  reserved[7];
 } nbody_server_config;
 
 typedef struct nbody_request_result {
@@ -362,12 +377,14 @@ typedef struct nbody_server_stats {
     uint64_t memory_used;      /* bytes held by admitted requests and idle handles */
     uint64_t memory_peak;
     uint64_t interactions;     /* of completed requests                           */
-    uint64_t ticks;
-    double   busy_seconds;     /* time spent in ticks with work                   */
+    uint64_t ticks;            /* ticks that completed or enqueued work            */
+    double   busy_seconds;     /* time spent in those ticks                       */
 } nbody_server_stats;
 
 /* law R, eps = 0.02, theta = 0.6, 64 GiB budget, 64 Mi-particle arena,
- * Barnes-Hut from 32768 particles, at most 64 Barnes-Hut requests, dt = 1e-3. */
+ * Barnes-Hut from 32768 particles, at most 16 Barnes-Hut requests with one
+ * step enqueued at a time (deeper queues delay the batch's steps: the streams
+ * share the device's hardware queues), dt = 1e-3. */
 void nbody_server_config_default(nbody_server_config* cfg);
 nbody_status nbody_server_create(const nbody_server_config* cfg, nbody_server** out);
 nbody_status nbody_server_destroy(nbody_server* server);
@@ -380,8 +397,13 @@ double nbody_server_clock(nbody_server* server);
 nbody_status nbody_server_submit_plummer(nbody_server* server, uint64_t seed, size_t n, int32_t steps, double a,
                                          double total_mass, double arrival, uint64_t* id);
 
-/* Admit, advance every active request one step, retire; *completed = requests
- * that finished in this tick. */
+/* One pass of the request loop, without waiting for the device: credit the
+ * steps that have completed (the batch's last step, each Barnes-Hut handle's
+ * enqueued steps), retire finished requests, admit queued ones that fit, and
+ * enqueue the next batch step and the next Barnes-Hut steps of idle handles.
+ * Each request advances as soon as its own work completes.  *completed =
+ * requests that finished in this tick.  Call it in a loop; a tick that finds
+ * nothing to do sleeps for about 100 microseconds. */
 nbody_status nbody_server_tick(nbody_server* server, int32_t* completed);
 
 /* Take up to max finished-request records (oldest first). */

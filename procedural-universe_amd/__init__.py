@@ -195,7 +195,8 @@ class ServerConfig(ctypes.Structure):
         ("max_active_bh", ctypes.c_int32),
         ("dt", ctypes.c_float),
         ("keep_results", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 8),
+        ("bh_steps_ahead", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
     ]
 
 
@@ -255,6 +256,8 @@ _SIGNATURES = {
     "nbody_init_soa": (ctypes.c_int, [_P, _F32P, _F32P, _F64P, ctypes.c_size_t]),
     "nbody_step": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_int32]),
     "nbody_synchronize": (ctypes.c_int, [_P]),
+    "nbody_query": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32)]),
+    "nbody_set_graph": (ctypes.c_int, [_P, ctypes.c_int32]),
     "nbody_read": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.c_uint32]),
     "nbody_read_soa": (ctypes.c_int, [_P, _F32P, _F32P, _F32P, ctypes.c_size_t]),
     "nbody_compute_accelerations": (ctypes.c_int, [_P, _F32P, ctypes.c_size_t]),
@@ -274,6 +277,7 @@ _SIGNATURES = {
     "nbody_batch_step": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_int32]),
     "nbody_batch_step_sequential": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_int32, ctypes.c_int32]),
     "nbody_batch_synchronize": (ctypes.c_int, [_P]),
+    "nbody_batch_query": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32)]),
     "nbody_batch_read": (ctypes.c_int, [_P, ctypes.c_int32, _F32P, _F32P, _F32P, ctypes.c_size_t]),
     "nbody_batch_get_info": (ctypes.c_int, [_P, ctypes.POINTER(BatchInfo)]),
     "nbody_batch_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(BatchTuning)]),
@@ -484,6 +488,16 @@ class NBodySim:
     def synchronize(self):
         _check("nbody_synchronize", self._lib.nbody_synchronize(self._h))
 
+    def query(self) -> bool:
+        """True when every enqueued step has completed (does not wait)."""
+        done = ctypes.c_int32(0)
+        _check("nbody_query", self._lib.nbody_query(self._h, ctypes.byref(done)))
+        return bool(done.value)
+
+    def set_graph(self, enable: bool = True):
+        """Replay step sequences from captured hipGraphs (bit-identical results)."""
+        _check("nbody_set_graph", self._lib.nbody_set_graph(self._h, 1 if enable else 0))
+
     def read_soa(self):
         n = self.n
         pos = np.empty((n, 3), np.float32)
@@ -607,6 +621,12 @@ class Batch:
 
     def synchronize(self):
         _check("nbody_batch_synchronize", self._lib.nbody_batch_synchronize(self._h))
+
+    def query(self) -> bool:
+        """True when every enqueued step has completed (does not wait)."""
+        done = ctypes.c_int32(0)
+        _check("nbody_batch_query", self._lib.nbody_batch_query(self._h, ctypes.byref(done)))
+        return bool(done.value)
 
     def read(self, sid: int):
         n = self.sizes[sid]

@@ -91,6 +91,13 @@ nbody_status dev_alloc(T** p, size_t count) {
     return NBODY_OK;
 }
 
+void drop_graphs(nbody_sim* s) {
+    for (auto& g : s->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = nbody_sim::StepGraph{};
+    }
+}
+
 nbody_status release_state(nbody_sim* s) {
     free_dev(s->pos[0]);
     free_dev(s->pos[1]);
@@ -122,6 +129,7 @@ DirectShape shape_for(const nbody_sim* s) {
 nbody_status prepare_state(nbody_sim* s, size_t n) {
     NB_HIP(hipSetDevice(s->cfg.device));
     NB_HIP(hipStreamSynchronize(s->stream));
+    drop_graphs(s);  // they hold the old sizes, pointers and launch arguments
     const size_t n_pad = round_up(std::max<size_t>(n, 1), kTile);
     if (n_pad > s->capacity) {
         release_state(s);
@@ -287,6 +295,7 @@ nbody_status nbody_destroy(nbody_sim* s) {
     if (!s) return NBODY_OK;
     (void)hipSetDevice(s->cfg.device);
     (void)hipStreamSynchronize(s->stream);
+    drop_graphs(s);
     if (s->bh) bh_destroy(s->bh);
     release_state(s);
     for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
@@ -341,18 +350,58 @@ nbody_status nbody_init_plummer(nbody_sim* s, uint64_t seed, size_t n, double a,
     NB_HIP(hipMemsetAsync(s->vel, 0, bytes, s->stream));
     NB_HIP(hipMemsetAsync(s->acc, 0, bytes, s->stream));
     NB_HIP(hipMemsetAsync(s->mass_dev, 0, s->n_pad * sizeof(double), s->stream));
+    NB_HIP(hipMemsetAsync(s->gm_dev, 0, s->n_pad * sizeof(double), s->stream));
+    // Everything is generated on the handle's stream; nothing waits for it
+    // here (steps and reads queue behind it).
     NB_HIP(ic_device_plummer(seed, static_cast<int>(n), a, total_mass, s->cfg.G, s->pos[0], s->vel, s->mass_dev,
-                             s->pos[1], s->ic_scratch, s->stream));
+                             s->gm_dev, s->pos[1], s->ic_scratch, s->stream));
     const double m = total_mass / static_cast<double>(n);
-    std::vector<double> hg(s->n_pad, 0.0);
-    std::fill(hg.begin(), hg.begin() + n, s->cfg.G * m);
-    NB_HIP(hipMemcpyAsync(s->gm_dev, hg.data(), s->n_pad * sizeof(double), hipMemcpyHostToDevice, s->stream));
-    NB_HIP(hipStreamSynchronize(s->stream));
     s->mass_host.assign(n, m);
     s->max_gm = std::fabs(s->cfg.G * m);
     s->initialised = true;
     s->acc_valid = false;
     if (s->bh) NB_TRY(bh_resize(s->bh, n, s));
+    return NBODY_OK;
+}
+
+// nbody_step as one graph launch: the counter reset and the nsteps force
+// passes are captured on the first call with this (nsteps, dt, parity) and
+// replayed afterwards.  The capture records kernels only (the counters are
+// cleared by a kernel, never a memset node).
+nbody_status step_graph(nbody_sim* s, float dt, int32_t nsteps) {
+    auto& G = s->graphs[s->cur];
+    const int parity = s->cur;
+    if (!(G.exec && G.steps == nsteps && G.dt == dt)) {
+        if (G.exec) (void)hipGraphExecDestroy(G.exec);
+        G = nbody_sim::StepGraph{};
+        hipGraph_t g = nullptr;
+        NB_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        nbody_status st = NBODY_OK;
+        if (s->bh) st = bh_reset_counters(s->bh, s->stream);
+        for (int k = 0; k < nsteps && st == NBODY_OK; ++k) st = force_pass(s, dt, true, nullptr, nullptr);
+        const hipError_t e = hipStreamEndCapture(s->stream, &g);
+        s->cur = parity;  // nothing ran yet: the launch below advances the state
+        if (st != NBODY_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        NB_HIP(e);
+        const hipError_t ei = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        NB_HIP(ei);
+        G.steps = nsteps;
+        G.dt = dt;
+    }
+    NB_HIP(hipGraphLaunch(G.exec, s->stream));
+    s->cur = parity ^ (nsteps & 1);
+    s->acc_valid = true;
+    s->step_interactions = s->bh ? 0 : static_cast<uint64_t>(nsteps) * s->n * s->n;
+    s->last.steps = nsteps;
+    s->last.interactions = s->step_interactions;
+    s->last.i_per_lane = s->shape.i_per_lane;
+    s->last.block = s->shape.block;
+    s->last.split = direct_effective_split(s->shape, static_cast<int>(s->n_pad));
+    s->timed_steps = 0;
     return NBODY_OK;
 }
 
@@ -370,6 +419,7 @@ nbody_status nbody_step(nbody_sim* s, float dt, int32_t nsteps) {
         for (hipEvent_t e : s->tree_ms_pending) (void)hipEventDestroy(e);
         s->tree_ms_pending.clear();
     }
+    if (s->use_graph && !s->timing && nsteps > 0 && s->n > 0) return step_graph(s, dt, nsteps);
     if (s->bh) NB_TRY(bh_reset_counters(s->bh, s->stream));
     if (s->n == 0) return NBODY_OK;
     if (s->timing) NB_TRY(ensure_events(s, static_cast<size_t>(nsteps) + 1));
@@ -395,6 +445,20 @@ nbody_status nbody_synchronize(nbody_sim* s) {
     NB_CHECK(s, NBODY_ERR_INVALID_ARG, "sim is null");
     NB_HIP(hipSetDevice(s->cfg.device));
     NB_HIP(hipStreamSynchronize(s->stream));
+    return NBODY_OK;
+}
+
+nbody_status nbody_query(nbody_sim* s, int32_t* done) {
+    clear_error();
+    NB_CHECK(s && done, NBODY_ERR_INVALID_ARG, "null argument");
+    NB_HIP(hipSetDevice(s->cfg.device));
+    const hipError_t e = hipStreamQuery(s->stream);
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return NBODY_OK;
+    }
+    NB_HIP(e);
+    *done = 1;
     return NBODY_OK;
 }
 
@@ -534,6 +598,16 @@ nbody_status nbody_diagnostics(nbody_sim* s, nbody_diag* out) {
     out->kinetic /= s->cfg.scale;
     out->potential = pot;
     out->total = out->kinetic + out->potential;
+    return NBODY_OK;
+}
+
+nbody_status nbody_set_graph(nbody_sim* s, int32_t enable) {
+    clear_error();
+    NB_CHECK(s, NBODY_ERR_INVALID_ARG, "sim is null");
+    NB_HIP(hipSetDevice(s->cfg.device));
+    NB_HIP(hipStreamSynchronize(s->stream));
+    s->use_graph = enable != 0;
+    if (!s->use_graph) drop_graphs(s);
     return NBODY_OK;
 }
 

@@ -182,11 +182,15 @@ struct nbody_batch {
     nbody_batch_tuning tuning{1, 256, 8, {0, 0, 0, 0, 0}, 0.0};
 
     bool use_graph = true;
-    hipGraphExec_t graph = nullptr;
-    int graph_steps = -1;
-    float graph_dt = 0.f;
-    int graph_parity = -1;
-    int graph_k = 0, graph_block = 0, graph_wg = 0;
+    // One captured graph per position-buffer parity (a step of an odd number
+    // of steps flips it), each for one (steps, dt, launch shape).
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        int steps = -1;
+        float dt = 0.f;
+        int k = 0, block = 0, wg = 0;
+    };
+    Graph graph[2];
 
     double* ic_scratch = nullptr;  // device initial conditions
 
@@ -289,9 +293,10 @@ nbody_status rebuild_items(nbody_batch* b) {
 }
 
 void drop_graph(nbody_batch* b) {
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
-    b->graph_steps = -1;
+    for (auto& g : b->graph) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = nbody_batch::Graph{};
+    }
 }
 
 BatchLaunch launch_for(nbody_batch* b, int parity, float dt) {
@@ -452,7 +457,7 @@ nbody_status nbody_batch_add_plummer(nbody_batch* b, uint64_t seed, size_t n, do
     if (e == hipSuccess) e = hipMemsetAsync(b->vel + off, 0, bytes, b->stream);
     if (e == hipSuccess)
         e = ic_device_plummer(seed, static_cast<int>(n), a, total_mass, b->cfg.G, b->pos[b->cur] + off, b->vel + off,
-                              nullptr, b->pos[b->cur ^ 1] + off, b->ic_scratch, b->stream);
+                              nullptr, nullptr, b->pos[b->cur ^ 1] + off, b->ic_scratch, b->stream);
     if (e != hipSuccess) {
         arena_free(b, off, n_pad);
         set_error("device initial conditions failed: %s", hipGetErrorString(e));
@@ -517,11 +522,12 @@ nbody_status nbody_batch_step(nbody_batch* b, float dt, int32_t nsteps) {
     }
     NB_HIP(hipEventRecord(b->ev[0], b->stream));
     if (b->use_graph) {
-        const bool reuse = b->graph && b->graph_steps == nsteps && b->graph_dt == dt && b->graph_parity == b->cur &&
-                           b->graph_k == b->tuning.i_per_lane && b->graph_block == b->tuning.block &&
-                           b->graph_wg == b->tuning.wg_per_cu;
+        auto& G = b->graph[b->cur];
+        const bool reuse = G.exec && G.steps == nsteps && G.dt == dt && G.k == b->tuning.i_per_lane &&
+                           G.block == b->tuning.block && G.wg == b->tuning.wg_per_cu;
         if (!reuse) {
-            drop_graph(b);
+            if (G.exec) (void)hipGraphExecDestroy(G.exec);
+            G = nbody_batch::Graph{};
             hipGraph_t g = nullptr;
             NB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             nbody_status st = enqueue_steps(b, dt, nsteps);
@@ -531,17 +537,16 @@ nbody_status nbody_batch_step(nbody_batch* b, float dt, int32_t nsteps) {
                 return st;
             }
             NB_HIP(e);
-            e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            e = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             NB_HIP(e);
-            b->graph_steps = nsteps;
-            b->graph_dt = dt;
-            b->graph_parity = b->cur;
-            b->graph_k = b->tuning.i_per_lane;
-            b->graph_block = b->tuning.block;
-            b->graph_wg = b->tuning.wg_per_cu;
+            G.steps = nsteps;
+            G.dt = dt;
+            G.k = b->tuning.i_per_lane;
+            G.block = b->tuning.block;
+            G.wg = b->tuning.wg_per_cu;
         }
-        NB_HIP(hipGraphLaunch(b->graph, b->stream));
+        NB_HIP(hipGraphLaunch(G.exec, b->stream));
     } else {
         NB_TRY(enqueue_steps(b, dt, nsteps));
     }
@@ -617,6 +622,20 @@ nbody_status nbody_batch_synchronize(nbody_batch* b) {
     NB_CHECK(b, NBODY_ERR_INVALID_ARG, "batch is null");
     NB_HIP(hipSetDevice(b->cfg.device));
     NB_HIP(hipStreamSynchronize(b->stream));
+    return NBODY_OK;
+}
+
+nbody_status nbody_batch_query(nbody_batch* b, int32_t* done) {
+    clear_error();
+    NB_CHECK(b && done, NBODY_ERR_INVALID_ARG, "null argument");
+    NB_HIP(hipSetDevice(b->cfg.device));
+    const hipError_t e = hipStreamQuery(b->stream);
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return NBODY_OK;
+    }
+    NB_HIP(e);
+    *done = 1;
     return NBODY_OK;
 }
 

@@ -44,6 +44,7 @@
 // approximates at the chain's deepest cell, which is the cell kept here.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -137,24 +138,24 @@ __device__ inline long long cell_index(float x, const RootParams& R, int a) {
 // decision.  The side depends only on the level, so one threshold per level.
 constexpr int kThrLevels = kBucketLevel + 1;
 
-__global__ void open_threshold_kernel(const RootParams* __restrict__ R, float theta_lo, float* __restrict__ thr) {
-    const int L = threadIdx.x;
-    if (L >= kThrLevels) return;
-    const float side = ldexpf(R->size, -L);
+__device__ float open_threshold(float side, float theta_lo) {
     auto approx = [&](unsigned bits) {
         return __fdiv_rn(side, __fsqrt_rn(__uint_as_float(bits))) <= theta_lo;
     };
     const unsigned kInf = 0x7f800000u;
-    if (!approx(kInf)) {
-        thr[L] = __uint_as_float(0x7fc00000u);  // NaN: r2 >= thr never holds
-        return;
-    }
-    unsigned lo = 0u, hi = kInf;  // !approx(lo), approx(hi)
+    if (!approx(kInf)) return __uint_as_float(0x7fc00000u);  // NaN: r2 >= thr never holds
+    unsigned lo = 0u, hi = kInf;                               // !approx(lo), approx(hi)
     while (hi - lo > 1u) {
         const unsigned mid = lo + (hi - lo) / 2u;
         if (approx(mid)) hi = mid; else lo = mid;
     }
-    thr[L] = __uint_as_float(hi);
+    return __uint_as_float(hi);
+}
+
+// Thresholds of a fixed root: computed once.
+__global__ void open_threshold_kernel(const RootParams* __restrict__ R, float theta_lo, float* __restrict__ thr) {
+    const int L = threadIdx.x;
+    if (L < kThrLevels) thr[L] = open_threshold(ldexpf(R->size, -L), theta_lo);
 }
 
 // ---- 1. root -------------------------------------------------------------
@@ -185,38 +186,56 @@ __global__ __launch_bounds__(256) void bounds_kernel(const float4* __restrict__ 
 }
 
 // Power-of-two cube over the bounding box with a corner on its level-21 grid
-// (same arithmetic as the CPU's oracle_dynamic_root).
-__global__ void dynamic_root_kernel(const float* __restrict__ partial, int nblocks, RootParams* __restrict__ R) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    for (int b = 0; b < nblocks; ++b)
+// (same arithmetic as the CPU's oracle_dynamic_root), then the opening
+// threshold of every level of that root.  One workgroup of 256 threads.
+__global__ __launch_bounds__(256) void dynamic_root_kernel(const float* __restrict__ partial, int nblocks,
+                                                           RootParams* __restrict__ R, float theta_lo,
+                                                           float* __restrict__ thr) {
+    __shared__ float red[6][256];
+    __shared__ float size_sh;
+    float pm[6] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};  // min x,y,z and -max x,y,z
+    for (int b = threadIdx.x; b < nblocks; b += 256)
         for (int a = 0; a < 3; ++a) {
-            mn[a] = fminf(mn[a], partial[b * 6 + a]);
-            mx[a] = fmaxf(mx[a], partial[b * 6 + 3 + a]);
+            pm[a] = fminf(pm[a], partial[b * 6 + a]);
+            pm[3 + a] = fminf(pm[3 + a], -partial[b * 6 + 3 + a]);
         }
-    float ext = 0.f;
-    for (int a = 0; a < 3; ++a) ext = fmaxf(ext, mx[a] - mn[a]);
-    int e;
-    frexpf(fmaxf(ext, 0x1p-100f), &e);
-    float size = ldexpf(1.0f, e);
-    RootParams out{};
-    for (int tries = 0; tries < 4; ++tries) {
-        const float cell = ldexpf(size, -kLevels);
-        bool fits = true;
-        for (int a = 0; a < 3; ++a) {
-            const long long k0 = static_cast<long long>(floorf(mn[a] / cell));
-            const long long k1 = static_cast<long long>(floorf(mx[a] / cell));
-            out.lo[a] = floorf(mn[a] / cell) * cell;
-            out.base[a] = k0;
-            if (k1 - k0 >= (1LL << kLevels)) fits = false;
-        }
-        if (fits) break;
-        size *= 2.f;
+    for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = pm[c];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = fminf(red[c][threadIdx.x], red[c][threadIdx.x + w]);
+        __syncthreads();
     }
-    out.size = size;
-    out.scale2 = ldexpf(1.0f, kLevels) / size;
-    out.odd = 1;
-    *R = out;
+    if (threadIdx.x == 0) {
+        const float mn[3] = {red[0][0], red[1][0], red[2][0]};
+        const float mx[3] = {-red[3][0], -red[4][0], -red[5][0]};
+        float ext = 0.f;
+        for (int a = 0; a < 3; ++a) ext = fmaxf(ext, mx[a] - mn[a]);
+        int e;
+        frexpf(fmaxf(ext, 0x1p-100f), &e);
+        float size = ldexpf(1.0f, e);
+        RootParams out{};
+        for (int tries = 0; tries < 4; ++tries) {
+            const float cell = ldexpf(size, -kLevels);
+            bool fits = true;
+            for (int a = 0; a < 3; ++a) {
+                const long long k0 = static_cast<long long>(floorf(mn[a] / cell));
+                const long long k1 = static_cast<long long>(floorf(mx[a] / cell));
+                out.lo[a] = floorf(mn[a] / cell) * cell;
+                out.base[a] = k0;
+                if (k1 - k0 >= (1LL << kLevels)) fits = false;
+            }
+            if (fits) break;
+            size *= 2.f;
+        }
+        out.size = size;
+        out.scale2 = ldexpf(1.0f, kLevels) / size;
+        out.odd = 1;
+        *R = out;
+        size_sh = size;
+    }
+    __syncthreads();
+    if (threadIdx.x < kThrLevels) thr[threadIdx.x] = open_threshold(ldexpf(size_sh, -threadIdx.x), theta_lo);
 }
 
 // ---- 2. keys ---------------------------------------------------------------
@@ -327,6 +346,54 @@ __global__ __launch_bounds__(kScanThreads2) void scan_apply_kernel(unsigned* __r
         if (i < total) data[i] = run;
         run += v[k];
     }
+}
+
+// The whole digit-count table in one workgroup when it is small (up to 64
+// counts per thread, each thread a contiguous run).
+constexpr int kScanSmallThreads = 1024;
+constexpr int kScanSmallMax = kScanSmallThreads * 64;
+
+__global__ __launch_bounds__(kScanSmallThreads) void scan_small_kernel(unsigned* __restrict__ data, int total) {
+    __shared__ unsigned wsum[kScanSmallThreads / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int per = (total + kScanSmallThreads - 1) / kScanSmallThreads;
+    const int b = min(t * per, total), e = min(b + per, total);
+    unsigned s = 0;
+    for (int i = b; i < e; ++i) s += data[i];
+    unsigned x = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    unsigned run = x - s;
+    for (int w = 0; w < wave; ++w) run += wsum[w];
+    for (int i = b; i < e; ++i) {
+        const unsigned v = data[i];
+        data[i] = run;
+        run += v;
+    }
+}
+
+// Counters: walk[3] (interactions, list entries, cells opened) and
+// counters[2] (outside, overflow) on the device, mirrored after every force
+// pass into pinned host memory (host[5], same order), so reading them needs
+// no copy: a copy is a blit kernel that would queue behind whatever occupies
+// the device.
+__global__ void zero_counters_kernel(unsigned* __restrict__ counters, unsigned long long* __restrict__ walk,
+                                     unsigned long long* __restrict__ host) {
+    if (threadIdx.x < 2) counters[threadIdx.x] = 0u;
+    if (threadIdx.x < 3) walk[threadIdx.x] = 0ULL;
+    if (threadIdx.x < 5) host[threadIdx.x] = 0ULL;
+}
+
+__global__ void publish_counters_kernel(const unsigned* __restrict__ counters,
+                                        const unsigned long long* __restrict__ walk,
+                                        unsigned long long* __restrict__ host) {
+    if (threadIdx.x < 3) host[threadIdx.x] = walk[threadIdx.x];
+    if (threadIdx.x >= 3 && threadIdx.x < 5) host[threadIdx.x] = counters[threadIdx.x - 3];
 }
 
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
@@ -947,6 +1014,7 @@ struct BHState {
     float* thr = nullptr;                    // [kThrLevels] opening thresholds
     unsigned* counters = nullptr;            // [0] outside, [1] overflow
     unsigned long long* walk = nullptr;      // [3] traversal counters
+    unsigned long long* host_ctr = nullptr;  // [5] pinned mirror of walk and counters
     int sorted_buf = 0;                      // which keys/idx buffer holds the sorted result
     RootParams fixed{};
     bool fixed_root = false;
@@ -1049,21 +1117,25 @@ nbody_status bh_create(nbody_sim* s) {
     hipError_t e1 = hipMalloc(reinterpret_cast<void**>(&b->root), sizeof(RootParams));
     hipError_t e2 = hipMalloc(reinterpret_cast<void**>(&b->counters), 2 * sizeof(unsigned));
     hipError_t e3 = hipMalloc(reinterpret_cast<void**>(&b->walk), 3 * sizeof(unsigned long long));
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+    hipError_t e4 = hipHostMalloc(reinterpret_cast<void**>(&b->host_ctr), 5 * sizeof(unsigned long long),
+                                  hipHostMallocDefault);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
         bh_destroy(b);
         set_error("hipMalloc failed for Barnes-Hut control blocks");
         return NBODY_ERR_OUT_OF_MEMORY;
     }
+    // On the handle's stream: synchronous copies would wait for whatever else
+    // occupies the device.
     if (b->fixed_root) {
-        hipError_t e = hipMemcpy(b->root, &b->fixed, sizeof(RootParams), hipMemcpyHostToDevice);
+        hipError_t e = hipMemcpyAsync(b->root, &b->fixed, sizeof(RootParams), hipMemcpyHostToDevice, s->stream);
         if (e != hipSuccess) {
             bh_destroy(b);
-            set_error("hipMemcpy failed: %s", hipGetErrorString(e));
+            set_error("hipMemcpyAsync failed: %s", hipGetErrorString(e));
             return NBODY_ERR_DEVICE;
         }
     }
-    (void)hipMemset(b->counters, 0, 2 * sizeof(unsigned));
-    (void)hipMemset(b->walk, 0, 3 * sizeof(unsigned long long));
+    for (int k = 0; k < 5; ++k) b->host_ctr[k] = 0;
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s->stream, b->counters, b->walk, b->host_ctr);
     s->bh = b;
     return NBODY_OK;
 }
@@ -1074,10 +1146,12 @@ void bh_destroy(BHState* b) {
     bh_free(b->root);
     bh_free(b->counters);
     bh_free(b->walk);
+    if (b->host_ctr) (void)hipHostFree(b->host_ctr);
+    b->host_ctr = nullptr;
     delete b;
 }
 
-nbody_status bh_resize(BHState* b, size_t n, nbody_sim*) {
+nbody_status bh_resize(BHState* b, size_t n, nbody_sim* s) {
     if (n <= b->cap && b->cap) return NBODY_OK;
     bh_release(b);
     const size_t m = n > 1 ? n - 1 : 1;
@@ -1105,6 +1179,12 @@ nbody_status bh_resize(BHState* b, size_t n, nbody_sim*) {
     NB_TRY(bh_alloc(b->com, m));
     NB_TRY(bh_alloc(b->thr, kThrLevels));
     b->cap = n;
+    if (b->fixed_root) {
+        // The fixed root's thresholds do not change: computed once per allocation.
+        hipLaunchKernelGGL(open_threshold_kernel, dim3(1), dim3(64), 0, s->stream, b->root,
+                           theta_below(s->cfg.theta), b->thr);
+        NB_HIP(hipGetLastError());
+    }
     return NBODY_OK;
 }
 
@@ -1114,8 +1194,10 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     hipStream_t st = s->stream;
     const float4* pos = s->pos[s->cur];
     if (!b->fixed_root) {
-        hipLaunchKernelGGL(bounds_kernel, dim3(kBoundsBlocks), dim3(256), 0, st, pos, n, b->bounds_partial);
-        hipLaunchKernelGGL(dynamic_root_kernel, dim3(1), dim3(64), 0, st, b->bounds_partial, kBoundsBlocks, b->root);
+        const int nbb = std::min(kBoundsBlocks, std::max(1, blocks_for(n, 2048)));
+        hipLaunchKernelGGL(bounds_kernel, dim3(nbb), dim3(256), 0, st, pos, n, b->bounds_partial);
+        hipLaunchKernelGGL(dynamic_root_kernel, dim3(1), dim3(256), 0, st, b->bounds_partial, nbb, b->root,
+                           theta_below(s->cfg.theta), b->thr);
     }
     hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, pos, n, b->root, b->keys[0],
                        b->idx[0], b->counters);
@@ -1125,9 +1207,15 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
         const int shift = 8 * pass;
         hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], n, shift, b->hist, nb);
         const int total = 256 * nb, nchunks = blocks_for(total, kScanChunk);
-        hipLaunchKernelGGL(scan_reduce_kernel, dim3(nchunks), dim3(kScanThreads2), 0, st, b->hist, total, b->chunk_sums);
-        hipLaunchKernelGGL(scan_chunks_kernel, dim3(1), dim3(kScanThreads2), 0, st, b->chunk_sums, nchunks);
-        hipLaunchKernelGGL(scan_apply_kernel, dim3(nchunks), dim3(kScanThreads2), 0, st, b->hist, total, b->chunk_sums);
+        if (total <= kScanSmallMax) {
+            hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(kScanSmallThreads), 0, st, b->hist, total);
+        } else {
+            hipLaunchKernelGGL(scan_reduce_kernel, dim3(nchunks), dim3(kScanThreads2), 0, st, b->hist, total,
+                               b->chunk_sums);
+            hipLaunchKernelGGL(scan_chunks_kernel, dim3(1), dim3(kScanThreads2), 0, st, b->chunk_sums, nchunks);
+            hipLaunchKernelGGL(scan_apply_kernel, dim3(nchunks), dim3(kScanThreads2), 0, st, b->hist, total,
+                               b->chunk_sums);
+        }
         hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], b->idx[cur],
                            b->keys[cur ^ 1], b->idx[cur ^ 1], n, shift, b->hist, nb);
         cur ^= 1;
@@ -1148,7 +1236,6 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     hipLaunchKernelGGL(mass_prefix_kernel, dim3(mchunks), dim3(kMassThreads), 0, st, M);
     hipLaunchKernelGGL(cell_com_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->prefix, n, b->first, b->last,
                        b->kept, m, s->cfg.G, b->com);
-    hipLaunchKernelGGL(open_threshold_kernel, dim3(1), dim3(64), 0, st, b->root, theta_below(s->cfg.theta), b->thr);
     LinkArgs L{keys, n, b->parent, b->leaf_parent, b->first, b->level, b->kept, b->com, b->thr, b->pos_sorted,
                b->blocks};
     hipLaunchKernelGGL(link_kernel, dim3(blocks_for(m + n, 256)), dim3(256), 0, st, L);
@@ -1177,6 +1264,8 @@ nbody_status bh_step(nbody_sim* s, float dt, bool kick, hipEvent_t ev_b, hipEven
         L.dt = dt;
         L.dts = static_cast<float>(static_cast<double>(dt) / s->cfg.scale);
         NB_HIP(direct_step(L, s->shape, kick, st, ev_b, ev_e));
+        hipLaunchKernelGGL(publish_counters_kernel, dim3(1), dim3(64), 0, st, b->counters, b->walk, b->host_ctr);
+        NB_HIP(hipGetLastError());
         return NBODY_OK;
     }
     hipEvent_t tb = nullptr, te = nullptr;
@@ -1217,26 +1306,25 @@ nbody_status bh_step(nbody_sim* s, float dt, bool kick, hipEvent_t ev_b, hipEven
                            b->first, b->last, A);
     NB_HIP(hipGetLastError());
     if (ev_e) NB_HIP(hipEventRecord(ev_e, st));
+    hipLaunchKernelGGL(publish_counters_kernel, dim3(1), dim3(64), 0, st, b->counters, b->walk, b->host_ctr);
+    NB_HIP(hipGetLastError());
     return NBODY_OK;
 }
 
 nbody_status bh_reset_counters(BHState* b, hipStream_t st) {
-    NB_HIP(hipMemsetAsync(b->counters, 0, 2 * sizeof(unsigned), st));
-    NB_HIP(hipMemsetAsync(b->walk, 0, 3 * sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, st, b->counters, b->walk, b->host_ctr);
+    NB_HIP(hipGetLastError());
     return NBODY_OK;
 }
 
 nbody_status bh_read_counters(BHState* b, hipStream_t st, BHCounters* out) {
-    unsigned c[2] = {0, 0};
-    unsigned long long w[3] = {0, 0, 0};
-    NB_HIP(hipMemcpyAsync(c, b->counters, sizeof c, hipMemcpyDeviceToHost, st));
-    NB_HIP(hipMemcpyAsync(w, b->walk, sizeof w, hipMemcpyDeviceToHost, st));
-    NB_HIP(hipStreamSynchronize(st));
-    out->interactions = w[0];
-    out->entries = w[1];
-    out->opened = w[2];
-    out->outside = c[0];
-    out->overflow = c[1];
+    NB_HIP(hipStreamSynchronize(st));  // the last force pass has published its counters
+    const volatile unsigned long long* h = b->host_ctr;
+    out->interactions = h[0];
+    out->entries = h[1];
+    out->opened = h[2];
+    out->outside = static_cast<unsigned>(h[3]);
+    out->overflow = static_cast<unsigned>(h[4]);
     return NBODY_OK;
 }
 

@@ -53,7 +53,7 @@ struct DStream {
 __global__ __launch_bounds__(kIcBlock) void plummer_kernel(unsigned long long seed, int n, double a, double m,
                                                            double G, float4* __restrict__ pos,
                                                            float4* __restrict__ vel, double* __restrict__ mass,
-                                                           double* __restrict__ partial) {
+                                                           double* __restrict__ gm, double* __restrict__ partial) {
     __shared__ double red[6][kIcBlock];
     double acc[6] = {0, 0, 0, 0, 0, 0};
     const double vscale = sqrt(G * (m * n) / a);
@@ -83,6 +83,7 @@ __global__ __launch_bounds__(kIcBlock) void plummer_kernel(unsigned long long se
         pos[i] = make_float4(px, py, pz, static_cast<float>(G * m));
         vel[i] = make_float4(vx, vy, vz, 0.f);
         if (mass) mass[i] = m;
+        if (gm) gm[i] = G * m;
         acc[0] += px, acc[1] += py, acc[2] += pz, acc[3] += vx, acc[4] += vy, acc[5] += vz;
     }
     for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = acc[c];
@@ -126,12 +127,13 @@ __global__ __launch_bounds__(kIcBlock) void recentre_kernel(int n, int nblocks, 
 size_t ic_device_scratch_doubles() { return 6 * kIcMaxBlocks; }
 
 hipError_t ic_device_plummer(unsigned long long seed, int n, double a, double total_mass, double G, float4* pos,
-                             float4* vel, double* mass, float4* pos_copy, double* scratch, hipStream_t stream) {
+                             float4* vel, double* mass, double* gm, float4* pos_copy, double* scratch,
+                             hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     int blocks = (n + kIcBlock - 1) / kIcBlock;
     if (blocks > kIcMaxBlocks) blocks = kIcMaxBlocks;
     hipLaunchKernelGGL(plummer_kernel, dim3(blocks), dim3(kIcBlock), 0, stream, seed, n, a, total_mass / n, G, pos,
-                       vel, mass, scratch);
+                       vel, mass, gm, scratch);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(recentre_kernel, dim3(blocks), dim3(kIcBlock), 0, stream, n, blocks, scratch, pos, vel,

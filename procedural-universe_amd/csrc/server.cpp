@@ -10,18 +10,21 @@
 //                 request may pass a large one that does not fit yet); a
 //                 request's state is generated on the device and stays there;
 //  * direct     - requests below the size threshold join the batch arena and
-//                 advance in its persistent hipGraph launch, one step per tick;
+//                 advance in its persistent hipGraph launch, one batch step
+//                 at a time;
 //  * Barnes-Hut - larger requests get a simulation handle of their own (from
-//                 a pool of idle handles), on its own stream, so their steps
-//                 overlap the batch's on the device;
+//                 a pool of idle handles), on its own stream, and a few steps
+//                 are enqueued at a time, so their steps overlap the batch's
+//                 and each other's on the device;
 //  * retirement - a request that has taken its steps leaves the batch (or
 //                 returns its handle), its memory is released and its
-//                 completion time recorded; requests that arrived meanwhile
-//                 are admitted on the next tick.
+//                 completion time recorded.
 //
-// A tick is one step of every active request.  Every request's arithmetic is
-// independent of the others (batch items and handles never mix systems), so a
-// result is the same alone and in the mix.
+// Nothing waits on the device: a tick credits the work that has completed
+// (stream queries), retires, admits, and enqueues more, so the batch steps at
+// its own pace and every Barnes-Hut request at its own.  Every request's
+// arithmetic is independent of the others (batch items and handles never mix
+// systems), so a result is the same alone and in the mix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,6 +34,7 @@
 #include <deque>
 #include <map>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -56,6 +60,8 @@ struct Request {
     double t_arrival = 0, t_admit = 0, t_done = 0;
     int32_t method = NBODY_METHOD_DIRECT;
     int32_t batch_id = -1;
+    int32_t inflight = 0;         // Barnes-Hut steps enqueued and not yet credited
+    bool in_batch_step = false;   // member of the batch step in flight
     nbody_sim* sim = nullptr;
     size_t sim_cap = 0;
     size_t footprint = 0;
@@ -81,6 +87,7 @@ struct nbody_server {
     std::vector<Idle> idle;
     std::map<uint64_t, std::vector<float>> kept;  // id -> pos[3n] then vel[3n]
     uint64_t next_id = 1;
+    bool batch_inflight = false;
     size_t mem_used = 0, mem_peak = 0;
     nbody_server_stats stats{};
 
@@ -142,6 +149,7 @@ nbody_status admit(nbody_server* s, Request& r, bool* ok) {
             nbody_config c = s->cfg.sim;
             c.method = NBODY_METHOD_BARNES_HUT;
             NB_TRY(nbody_create(&c, &r.sim));
+            NB_TRY(nbody_set_graph(r.sim, 1));  // a few graph launches per tick instead of ~40 kernels a step
             r.sim_cap = r.n;
             s->mem_used += f;
         }
@@ -207,9 +215,10 @@ void nbody_server_config_default(nbody_server_config* c) {
     c->memory_budget = 64ull << 30;
     c->direct_capacity = 64ull << 20;
     c->bh_threshold = 32768;
-    c->max_active_bh = 64;
+    c->max_active_bh = 16;
     c->dt = 1e-3f;
     c->keep_results = 0;
+    c->bh_steps_ahead = 1;
 }
 
 nbody_status nbody_server_create(const nbody_server_config* cfg, nbody_server** out) {
@@ -273,7 +282,51 @@ nbody_status nbody_server_tick(nbody_server* s, int32_t* completed) {
     NB_CHECK(s, NBODY_ERR_INVALID_ARG, "server is null");
     if (completed) *completed = 0;
     const uint64_t done_before = s->stats.completed;
-    // 1. admission, first fit in arrival order
+    const double t_start = s->clock();
+    bool progressed = false;
+    // 1. credit completed work
+    if (s->batch_inflight) {
+        int32_t idle = 0;
+        NB_TRY(nbody_batch_query(s->batch, &idle));
+        if (idle) {
+            for (auto& r : s->active)
+                if (r.in_batch_step) {
+                    r.in_batch_step = false;
+                    r.done += 1;
+                    r.interactions += static_cast<uint64_t>(r.n) * r.n;
+                }
+            s->batch_inflight = false;
+            progressed = true;
+        }
+    }
+    for (auto& r : s->active) {
+        if (r.method != NBODY_METHOD_BARNES_HUT || r.inflight == 0) continue;
+        int32_t idle = 0;
+        NB_TRY(nbody_query(r.sim, &idle));
+        if (!idle) continue;
+        nbody_timing t;
+        NB_TRY(nbody_get_timing(r.sim, &t));
+        r.interactions += t.interactions;
+        r.done += r.inflight;
+        r.inflight = 0;
+        progressed = true;
+    }
+    // 2. retirement (a direct request only while the batch is idle: leaving
+    // the arena changes its items)
+    for (size_t k = 0; k < s->active.size();) {
+        const Request& r = s->active[k];
+        const bool finished = r.done >= r.steps && r.inflight == 0 && !r.in_batch_step &&
+                              (r.method != NBODY_METHOD_DIRECT || !s->batch_inflight);
+        if (finished) {
+            NB_TRY(retire(s, s->active[k]));
+            s->active.erase(s->active.begin() + k);
+            progressed = true;
+        } else {
+            ++k;
+        }
+    }
+    // 3. admission, first fit in arrival order (a direct request added while
+    // a batch step runs joins from the next step on)
     for (auto it = s->pending.begin(); it != s->pending.end();) {
         bool ok = false;
         NB_TRY(admit(s, *it, &ok));
@@ -281,52 +334,48 @@ nbody_status nbody_server_tick(nbody_server* s, int32_t* completed) {
             s->active.push_back(*it);
             s->stats.admitted += 1;
             it = s->pending.erase(it);
+            progressed = true;
         } else {
             ++it;
         }
     }
     // Requests of zero steps complete on admission.
     for (size_t k = 0; k < s->active.size();) {
-        if (s->active[k].done >= s->active[k].steps) {
+        const Request& r = s->active[k];
+        if (r.steps == 0 && (r.method != NBODY_METHOD_DIRECT || !s->batch_inflight)) {
             NB_TRY(retire(s, s->active[k]));
             s->active.erase(s->active.begin() + k);
         } else {
             ++k;
         }
     }
-    if (s->active.empty()) {
-        if (completed) *completed = static_cast<int32_t>(s->stats.completed - done_before);
-        return NBODY_OK;
+    // 4. enqueue: the next batch step, and the next steps of idle handles
+    if (!s->batch_inflight) {
+        bool any = false;
+        for (auto& r : s->active)
+            if (r.method == NBODY_METHOD_DIRECT && r.done < r.steps) {
+                r.in_batch_step = true;
+                any = true;
+            }
+        if (any) {
+            NB_TRY(nbody_batch_step(s->batch, s->cfg.dt, 1));
+            s->batch_inflight = true;
+            progressed = true;
+        }
     }
-    const double t_start = s->clock();
-    // 2. one step of every active request: the batch graph and each handle's
-    // stream run concurrently on the device.
-    bool any_direct = false;
-    for (const auto& r : s->active) any_direct |= (r.method == NBODY_METHOD_DIRECT);
-    if (any_direct) NB_TRY(nbody_batch_step(s->batch, s->cfg.dt, 1));
+    const int32_t ahead = s->cfg.bh_steps_ahead > 0 ? s->cfg.bh_steps_ahead : 1;
     for (auto& r : s->active)
-        if (r.method == NBODY_METHOD_BARNES_HUT) NB_TRY(nbody_step(r.sim, s->cfg.dt, 1));
-    if (any_direct) NB_TRY(nbody_batch_synchronize(s->batch));
-    for (auto& r : s->active) {
-        if (r.method == NBODY_METHOD_BARNES_HUT) {
-            nbody_timing t;
-            NB_TRY(nbody_get_timing(r.sim, &t));
-            r.interactions += t.interactions;
-        } else {
-            r.interactions += static_cast<uint64_t>(r.n) * r.n;
+        if (r.method == NBODY_METHOD_BARNES_HUT && r.inflight == 0 && r.done < r.steps) {
+            const int32_t k = std::min(ahead, r.steps - r.done);
+            NB_TRY(nbody_step(r.sim, s->cfg.dt, k));
+            r.inflight = k;
+            progressed = true;
         }
-        r.done += 1;
-    }
-    s->stats.ticks += 1;
-    s->stats.busy_seconds += s->clock() - t_start;
-    // 3. retirement
-    for (size_t k = 0; k < s->active.size();) {
-        if (s->active[k].done >= s->active[k].steps) {
-            NB_TRY(retire(s, s->active[k]));
-            s->active.erase(s->active.begin() + k);
-        } else {
-            ++k;
-        }
+    if (progressed) {
+        s->stats.ticks += 1;
+        s->stats.busy_seconds += s->clock() - t_start;
+    } else {
+        std::this_thread::sleep_for(std::chrono::microseconds(100));
     }
     if (completed) *completed = static_cast<int32_t>(s->stats.completed - done_before);
     return NBODY_OK;

@@ -50,4 +50,14 @@ struct nbody_sim {
     int timed_steps = 0;
     uint64_t step_interactions = 0;
     nbody_timing last{};
+
+    // nbody_set_graph: an nbody_step sequence is captured once and replayed,
+    // one graph per parity of the current positions.
+    bool use_graph = false;
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        int32_t steps = -1;
+        float dt = 0.f;
+    };
+    StepGraph graphs[2];
 };
