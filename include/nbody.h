@@ -253,6 +253,28 @@ nbody_status nbody_bh_stats_get(nbody_sim* sim, nbody_bh_stats* out);
  * build, for checking against a CPU computation.  Either pointer may be NULL. */
 nbody_status nbody_bh_tree_order(nbody_sim* sim, uint64_t* keys, uint32_t* order, size_t n);
 
+/* One octree cell of the last tree build. */
+typedef struct nbody_bh_cell {
+    float    lo[3];     /* corner of the cube: root corner + index * side        */
+    float    side;      /* root size / 2^level                                   */
+    float    com[3];    /* centre of mass as the traversal uses it (FP32)        */
+    uint32_t level;     /* 0 = root; 21 = a cell of identical keys               */
+    double   mass;      /* FP64 sum of its particles' masses                     */
+    uint32_t first;     /* its particles: sorted positions [first, first+count)  */
+    uint32_t count;     /* (see nbody_bh_tree_order for the order)                */
+} nbody_bh_cell;
+
+/* The cells of the last tree build, for drawing or checking the octree (the
+ * reference's Octree::RenderDebug draws its cubes, src/Sim/Octree.cpp:147-175,
+ * called from BarnesHut::RenderDebug, src/Sim/BarnesHut.cpp:98-101).  The
+ * octree is compressed: a cell is listed where its particles split among two
+ * or more children (or at the root); the reference's one-particle leaves and
+ * the chains of cells with a single non-empty child above a split are not
+ * listed.  Cells come in depth-first order of the binary radix tree.  *count
+ * receives the number of cells; at most capacity are written (out may be NULL
+ * with capacity 0 to ask for the count). */
+nbody_status nbody_bh_cells(nbody_sim* sim, nbody_bh_cell* out, size_t capacity, size_t* count);
+
 /* ------------------------------------------------------------------------ */
 /* Batched direct sum: many independent systems in one device arena          */
 /* ------------------------------------------------------------------------ */

@@ -44,6 +44,13 @@ PARTICLE_DTYPE = np.dtype(
 )
 assert PARTICLE_DTYPE.itemsize == 104
 
+# nbody_bh_cell (include/nbody.h): one octree cell, 48 bytes.
+BH_CELL_DTYPE = np.dtype([
+    ("lo", np.float32, 3), ("side", np.float32), ("com", np.float32, 3), ("level", np.uint32),
+    ("mass", np.float64), ("first", np.uint32), ("count", np.uint32),
+])
+assert BH_CELL_DTYPE.itemsize == 48
+
 # nbody_status
 OK = 0
 STATUS_NAMES = {
@@ -266,6 +273,7 @@ _SIGNATURES = {
     "nbody_set_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
     "nbody_get_timing": (ctypes.c_int, [_P, ctypes.POINTER(Timing)]),
     "nbody_bh_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(BHStats)]),
+    "nbody_bh_cells": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "nbody_bh_tree_order": (
         ctypes.c_int,
         [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32), ctypes.c_size_t],
@@ -545,6 +553,17 @@ class NBodySim:
             ),
         )
         return keys, order
+
+    def bh_cells(self) -> np.ndarray:
+        """Octree cells of the last Barnes-Hut tree build (structured array of
+        lo[3], side, com[3], level, mass, first, count), depth-first."""
+        count = ctypes.c_size_t(0)
+        _check("nbody_bh_cells", self._lib.nbody_bh_cells(self._h, None, 0, ctypes.byref(count)))
+        out = np.zeros(count.value, dtype=BH_CELL_DTYPE)
+        got = ctypes.c_size_t(0)
+        _check("nbody_bh_cells", self._lib.nbody_bh_cells(self._h, out.ctypes.data_as(ctypes.c_void_p),
+                                                          count.value, ctypes.byref(got)))
+        return out
 
     def set_timing(self, enable: bool = True):
         _check("nbody_set_timing", self._lib.nbody_set_timing(self._h, 1 if enable else 0))

@@ -673,6 +673,15 @@ nbody_status nbody_bh_tree_order(nbody_sim* s, uint64_t* keys, uint32_t* order, 
     return bh_export_order(s, keys, order, n);
 }
 
+nbody_status nbody_bh_cells(nbody_sim* s, nbody_bh_cell* out, size_t capacity, size_t* count) {
+    clear_error();
+    NB_CHECK(s && count && (out || capacity == 0), NBODY_ERR_INVALID_ARG, "bad argument");
+    NB_CHECK(s->bh, NBODY_ERR_UNSUPPORTED, "not a Barnes-Hut simulation");
+    NB_CHECK(s->initialised, NBODY_ERR_NOT_INITIALISED, "no tree before nbody_init");
+    NB_HIP(hipSetDevice(s->cfg.device));
+    return bh_export_cells(s, out, capacity, count);
+}
+
 const char* nbody_method_name(int32_t method) {
     switch (method) {
         case NBODY_METHOD_DIRECT: return "Brute Force GPU";

@@ -752,6 +752,46 @@ __global__ __launch_bounds__(256) void cell_com_kernel(const DD* __restrict__ pr
     com[i] = make_float4(cx, cy, cz, static_cast<float>(G * r[0]));
 }
 
+// The exported description of every radix-tree node that is an octree cell
+// (the others are left unwritten; the host keeps those with kept != 0).
+struct CellExport {
+    const unsigned long long* keys;  // sorted
+    const int* first;
+    const int* last;
+    const int* level;
+    const int* kept;
+    const float4* com;
+    const DD* prefix;  // [4][n + 1]
+    const RootParams* root;
+    int n;
+    nbody_bh_cell* out;
+};
+
+__global__ __launch_bounds__(256) void export_cells_kernel(CellExport E) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= E.n - 1 || !E.kept[i]) return;
+    const RootParams R = *E.root;
+    const int L = E.level[i], f = E.first[i], l = E.last[i];
+    const unsigned long long cell = E.keys[f] >> (3 * (kLevels - L));  // the key's first L groups
+    nbody_bh_cell c{};
+    const double side = ldexp(static_cast<double>(R.size), -L);
+    for (int a = 0; a < 3; ++a) {
+        unsigned long long idx = 0;
+        for (int g = 0; g < L; ++g) idx |= ((cell >> (3 * g + a)) & 1ULL) << g;
+        c.lo[a] = static_cast<float>(static_cast<double>(R.lo[a]) + static_cast<double>(idx) * side);
+    }
+    c.side = static_cast<float>(side);
+    const float4 m = E.com[i];
+    c.com[0] = m.x;
+    c.com[1] = m.y;
+    c.com[2] = m.z;
+    c.level = static_cast<unsigned>(L);
+    c.mass = dd_diff(E.prefix[l + 1], E.prefix[f]);
+    c.first = static_cast<unsigned>(f);
+    c.count = static_cast<unsigned>(l - f + 1);
+    E.out[i] = c;
+}
+
 __global__ __launch_bounds__(256) void gather_kernel(const float4* __restrict__ pos, const unsigned* __restrict__ order,
                                                      int n, float4* __restrict__ out) {
     const int k = blockIdx.x * 256 + threadIdx.x;
@@ -823,6 +863,21 @@ __device__ __forceinline__ void flush_list(const float4* lsrc, const uint2* lmas
     kahan_add(sz, cz, az);
 }
 
+// Workgroups are dealt to the accelerator's XCDs in turn (workgroup b to XCD
+// b mod 8), and each XCD has its own L2.  Neighbouring particle groups in
+// Morton order walk nearly the same cells, so the groups are renumbered to
+// give every XCD one contiguous eighth of the order: its L2 then holds the
+// child blocks its waves share.  A bijection on [0, nb).
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    constexpr int C = 8;
+    const int full = (nb / (kXcds * C)) * (kXcds * C);
+    if (b >= full) return b;
+    const int x = b % kXcds, k = b / kXcds;
+    return ((k / C) * kXcds + x) * C + k % C;
+}
+
 template <int LAW>
 __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __restrict__ pos_sorted,
                                                                 const ChildBlock* __restrict__ blocks,
@@ -835,8 +890,9 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
     __shared__ unsigned long long red[3][kTravWaves];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: list and stack addresses in SGPRs
-    const int t = blockIdx.x * kTravThreads + tid;
-    const int wave_base = blockIdx.x * kTravThreads + wave * 64;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int t = blk * kTravThreads + tid;
+    const int wave_base = blk * kTravThreads + wave * 64;
     const bool valid = t < A.n;
     const float4 me = pos_sorted[valid ? t : A.n - 1];
     const float soft = core::in_vgpr(A.soft);
@@ -939,8 +995,8 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         }
 #pragma unroll
         for (int slot = 0; slot < 8; ++slot)
-            asm volatile("" : "+s"(srcs[slot].x), "+s"(srcs[slot].y), "+s"(srcs[slot].z), "+s"(srcs[slot].w),
-                         "+s"(thrs[slot]), "+s"(codes[slot]));
+            asm volatile("" ::"s"(srcs[slot].x), "s"(srcs[slot].y), "s"(srcs[slot].z), "s"(srcs[slot].w),
+                         "s"(thrs[slot]), "s"(codes[slot]));
 #pragma unroll
         for (int slot = 0; slot < 8; ++slot) {
             const int c = codes[slot];
@@ -1018,6 +1074,7 @@ struct BHState {
     int sorted_buf = 0;                      // which keys/idx buffer holds the sorted result
     RootParams fixed{};
     bool fixed_root = false;
+    bool built = false;  // a tree has been built since the buffers were sized
 };
 
 namespace {
@@ -1054,6 +1111,7 @@ void bh_release(BHState* b) {
     bh_free(b->thr);
     b->cap = 0;
     b->hist_elems = 0;
+    b->built = false;
 }
 
 template <class T>
@@ -1238,6 +1296,7 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
                        b->kept, m, s->cfg.G, b->com);
     LinkArgs L{keys, n, b->parent, b->leaf_parent, b->first, b->level, b->kept, b->com, b->thr, b->pos_sorted,
                b->blocks};
+    b->built = true;
     hipLaunchKernelGGL(link_kernel, dim3(blocks_for(m + n, 256)), dim3(256), 0, st, L);
     NB_HIP(hipGetLastError());
     return NBODY_OK;
@@ -1325,6 +1384,39 @@ nbody_status bh_read_counters(BHState* b, hipStream_t st, BHCounters* out) {
     out->opened = h[2];
     out->outside = static_cast<unsigned>(h[3]);
     out->overflow = static_cast<unsigned>(h[4]);
+    return NBODY_OK;
+}
+
+nbody_status bh_export_cells(nbody_sim* s, nbody_bh_cell* out, size_t capacity, size_t* count) {
+    BHState* b = s->bh;
+    const size_t n = s->n;
+    NB_CHECK(b && b->built && b->cap >= n, NBODY_ERR_NOT_INITIALISED, "no Barnes-Hut tree built yet");
+    *count = 0;
+    if (n < 2) return NBODY_OK;
+    const size_t m = n - 1;
+    nbody_bh_cell* d = nullptr;
+    NB_HIP(hipMalloc(reinterpret_cast<void**>(&d), m * sizeof(nbody_bh_cell)));
+    CellExport E{b->keys[b->sorted_buf], b->first, b->last, b->level, b->kept, b->com, b->prefix, b->root,
+                 static_cast<int>(n), d};
+    hipLaunchKernelGGL(export_cells_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, s->stream, E);
+    std::vector<nbody_bh_cell> cells(m);
+    std::vector<int> kept(m);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(cells.data(), d, m * sizeof(nbody_bh_cell), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(kept.data(), b->kept, m * sizeof(int), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(d);
+    NB_HIP(e);
+    std::vector<nbody_bh_cell> list;
+    for (size_t i = 0; i < m; ++i)
+        if (kept[i]) list.push_back(cells[i]);
+    // depth-first preorder of the octree: by first particle, enclosing cells first
+    std::sort(list.begin(), list.end(), [](const nbody_bh_cell& x, const nbody_bh_cell& y) {
+        return x.first != y.first ? x.first < y.first : x.count > y.count;
+    });
+    *count = list.size();
+    for (size_t i = 0; i < list.size() && i < capacity; ++i) out[i] = list[i];
     return NBODY_OK;
 }
 

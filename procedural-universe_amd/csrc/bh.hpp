@@ -35,5 +35,7 @@ nbody_status bh_reset_counters(BHState* b, hipStream_t st);
 nbody_status bh_read_counters(BHState* b, hipStream_t st, BHCounters* out);
 // Sorted keys and particle order of the last tree build.
 nbody_status bh_export_order(nbody_sim* s, uint64_t* keys, uint32_t* order, size_t n);
+// Octree cells of the last tree build.
+nbody_status bh_export_cells(nbody_sim* s, nbody_bh_cell* out, size_t capacity, size_t* count);
 
 }  // namespace nbody

@@ -215,3 +215,70 @@ def test_opening_decision_at_the_boundary(gpu, oracle, k):
     F, _ = oracle.bh_forces(pos, mass, 0.5, REF_LO, REF_SIZE)
     err = rel_err_rows(a, F / mass[:, None])
     assert err.max() < 1e-5, err
+
+
+def test_cell_export_describes_the_octree(gpu):
+    """nbody_bh_cells: every cell is a cube of its level on the root's grid that
+    holds exactly its particles, with their FP64 mass and centre of mass, and
+    (below level 21) splits them among two or more octants."""
+    pu = gpu
+    n = 20000
+    pos, vel, mass = pu.ic_plummer(n, 0x5EED0009)
+    cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=1e-4, theta=0.5)
+    with _bh(pu, cfg, pos, vel, mass) as sim:
+        sim.accelerations()
+        keys, order = sim.bh_tree_order()
+        cells = sim.bh_cells()
+    assert cells.size > n // 8
+    root = cells[0]
+    assert root["level"] == 0 and root["first"] == 0 and root["count"] == n
+    p = pos[order].astype(np.float64)
+    m = mass[order]
+    for c in cells[:: max(1, cells.size // 2000)]:
+        f, k = int(c["first"]), int(c["count"])
+        q = p[f:f + k]
+        lo = c["lo"].astype(np.float64)
+        assert np.all(q >= lo) and np.all(q < lo + float(c["side"]))
+        assert float(c["side"]) == float(root["side"]) * 2.0 ** -int(c["level"])
+        M = m[f:f + k].sum()
+        assert c["mass"] == pytest.approx(M, rel=1e-12)
+        com = (q * m[f:f + k, None]).sum(0) / M
+        assert np.all(np.abs(c["com"] - com) <= 1e-6 * float(root["side"]))
+        if c["level"] < 21:
+            octant = (((q - lo) / (float(c["side"]) / 2)).astype(np.int64) * np.array([1, 2, 4])).sum(1)
+            assert np.unique(octant).size >= 2
+    # depth-first preorder: by first particle, enclosing cells first
+    first = cells["first"].astype(np.int64)
+    count = cells["count"].astype(np.int64)
+    assert np.all(np.diff(first) >= 0)
+    same = np.diff(first) == 0
+    assert np.all(np.diff(count)[same] < 0)
+
+
+def test_full_size_encounter_keys_and_tree(gpu, oracle):
+    """N = 4,194,304 (two 2,097,152-particle discs on a parabolic encounter,
+    theta 0.5): keys and stable order bit-exact against the CPU on the full N,
+    no stack overflow, the root cell holds every particle and the tree's FP64
+    total mass equals the particles'."""
+    pu = gpu
+    n = 1 << 22
+    pos, vel, mass = pu.ic_disc_encounter(n, 0x5EED0003, 3.0, 0.3, 40.0, 5.0, 30.0)
+    cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=0.0025, theta=0.5)
+    with _bh(pu, cfg, pos, vel, mass) as sim:
+        sim.step(5e-4, 1)
+        st = sim.bh_stats()
+        keys, order = sim.bh_tree_order()  # the tree of the last step's state
+        p1, _, _ = sim.read_soa()
+        cells = sim.bh_cells()
+    lo, size = oracle.dynamic_root(pos)
+    ref_keys, outside = oracle.morton_keys(pos, lo, size)
+    ref_order = oracle.sort_order(ref_keys)
+    assert outside == 0
+    assert np.array_equal(order, ref_order)
+    assert np.array_equal(keys, ref_keys[ref_order])
+    assert st["stack_overflow"] == 0
+    ipp = st["interactions"] / n
+    print(f"full size: {ipp:.0f} interactions/particle, {cells.size} cells")
+    assert 300 < ipp < 5000
+    assert cells[0]["count"] == n and cells[0]["mass"] == pytest.approx(mass.sum(), rel=1e-12)
+    assert np.all(np.isfinite(p1))
