@@ -34,6 +34,12 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector peak (AMD public spec)
 # default shape, from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a 256 MiB
 # copy (profiles/direct_force_n1m_law_r_traffic.md).
 DIRECT_1M_TRAFFIC_BYTES = 2.1854e9
+# The same for the Barnes-Hut traversal step launch at N = 2^22, theta 0.5
+# (profiles/bh_traverse_n4m_theta05_traffic.md) and the 512-galaxy batch step
+# (profiles/batch_step_512_galaxies_traffic.md); they hold for the default
+# workloads only.
+BH_4M_TRAFFIC_BYTES = 5.1668e9
+BATCH_512_TRAFFIC_BYTES = 1.4884e9
 FLOP_PER_INTERACTION = 20  # Nyland, Harris & Prins (GPU Gems 3, ch. 31) convention
 METRIC = "interactions/s & % FP32 VALU peak at N=1M; batched p50/p99 step latency"
 
@@ -80,6 +86,8 @@ def bench_batch(pu, args):
     exponential disc) advanced together by the persistent batch kernel in one
     hipGraph launch, against each galaxy alone with plain launches."""
     count, steps = args.galaxies, args.steps
+    # the measured traffic belongs to the default workload only
+    default_batch = count == 512 and args.seed == 0x5EED0004 and args.eps == 0.01 and args.law == "R"
     t0 = time.time()
     gal = pu.batch_galaxies(count, args.seed, 1024, 65536)
     t_ic = time.time() - t0
@@ -160,7 +168,9 @@ def bench_batch(pu, args):
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": FLOP_PER_INTERACTION * value / 1e12 / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": BATCH_512_TRAFFIC_BYTES if default_batch else None,
+            "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (256 MiB copy calibration), "
+            "profiles/batch_step_512_galaxies_traffic.md" if default_batch else None,
         },
     }
 
@@ -171,6 +181,7 @@ def bench_bh(pu, args):
     disc tilted 30 degrees)."""
     n = args.n
     soft = args.eps * args.eps
+    default_bh = n == 1 << 22 and args.theta == 0.5 and args.seed == 0x5EED0003 and args.eps == 0.05
     t0 = time.time()
     pos, vel, mass = pu.ic_disc_encounter(n, args.seed, 3.0, 0.3, 40.0, 5.0, 30.0)
     t_ic = time.time() - t0
@@ -241,7 +252,9 @@ def bench_bh(pu, args):
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": FLOP_PER_INTERACTION * interactions / (args.steps * force_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": BH_4M_TRAFFIC_BYTES if default_bh else None,
+            "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (256 MiB copy calibration), "
+            "profiles/bh_traverse_n4m_theta05_traffic.md" if default_bh else None,
             "kernel": "traverse_kernel",
         },
     }
