@@ -146,7 +146,10 @@ nbody_status prepare_state(nbody_sim* s, size_t n) {
     s->cur = 0;
     s->shape = shape_for(s);
     const size_t split = static_cast<size_t>(direct_effective_split(s->shape, static_cast<int>(n_pad)));
-    const size_t need_part = std::max<size_t>(split, 1) * n_pad;
+    // Partial sums serve the direct kernel only (a Barnes-Hut handle uses it
+    // for a single particle alone).
+    const bool direct = s->cfg.method == NBODY_METHOD_DIRECT || n < 2;
+    const size_t need_part = direct ? std::max<size_t>(split, 1) * n_pad : 0;
     if (need_part > s->part_elems) {
         NB_TRY(dev_alloc(&s->part, need_part));
         s->part_elems = need_part;

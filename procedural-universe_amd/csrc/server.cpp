@@ -44,10 +44,11 @@ namespace nbody {
 namespace {
 
 // Device bytes per particle: a simulation handle (positions x2, velocity,
-// acceleration, partial sums, FP64 mass and G m) plus the Barnes-Hut tree
-// (keys and indices x2, sorted positions, radix-tree nodes, FP64 sums,
-// octree records).
-constexpr size_t kBhBytesPerParticle = 96 + 256;
+// acceleration, FP64 mass and G m: 80 B) plus the Barnes-Hut tree (keys and
+// indices x2: 24 B, sorted positions: 16 B, radix-tree node arrays: 32 B,
+// child blocks: 192 B, double-double prefix sums: 64 B, cell records: 16 B,
+// digit counts: under 1 B): 425 B, counted as 432.
+constexpr size_t kBhBytesPerParticle = 80 + 352;
 constexpr size_t kDirectBytesPerParticle = 64;
 
 struct Request {

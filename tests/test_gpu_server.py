@@ -68,7 +68,7 @@ def test_server_admission_respects_budget(gpu):
     pu = gpu
     reqs = [(4000, 3, k) for k in range(12)] + [(20000, 2, 99)]
     # room for about four direct requests at a time (64 B per particle)
-    budget = 4 * 4096 * 64 + 20480 * 352
+    budget = 4 * 4096 * 64 + 20480 * 432
     ids, out, results, peak, st = _serve(pu, reqs, budget=budget, threshold=10000)
     assert st["completed"] == len(reqs)
     assert peak <= budget
