@@ -28,6 +28,9 @@ static_assert(sizeof(nbody_particle) == 104, "nbody_particle must match the refe
 static_assert(offsetof(nbody_particle, velocity) == 48, "velocity offset");
 static_assert(offsetof(nbody_particle, forces) == 72, "forces offset");
 static_assert(offsetof(nbody_particle, mass) == 96, "mass offset");
+static_assert(sizeof(nbody_bh_cell) == 48 && offsetof(nbody_bh_cell, mass) == 32 &&
+                  offsetof(nbody_bh_cell, count) == 44,
+              "nbody_bh_cell layout (mirrored by BH_CELL_DTYPE in the Python package)");
 
 namespace nbody {
 
