@@ -829,8 +829,17 @@ using core::kahan_add;
 // opened the parent but its own.  When the list fills, every lane evaluates
 // all entries in a tight loop - four sources at a time with grouped rsq, the
 // source's G*m zeroed for lanes outside the entry - and adds the result to its
-// compensated sum.
-constexpr int kListCap = 128;
+// compensated sum.  A child's entry is written by the lane that holds the
+// child's record in vector registers (one vector load of the block per pop),
+// so an append moves only the address and the lane mask.
+constexpr int kListCap = 96;
+
+// A list entry: the source (x, y, z, G m) and the lanes it applies to.
+struct ListEntry {
+    float4 src;
+    uint2 mask;
+    uint2 pad;
+};
 
 // G*m where the lane's bit of the entry mask is set, else 0 (the bit
 // sign-extended to a full-width and-mask).
@@ -840,7 +849,7 @@ __device__ __forceinline__ float lane_gm(float gm, unsigned lo, unsigned hi, int
 }
 
 template <int LAW>
-__device__ __forceinline__ void flush_list(const float4* lsrc, const uint2* lmask, int cnt, int lane, float px,
+__device__ __forceinline__ void flush_list(const ListEntry* lst, int cnt, int lane, float px,
                                            float py, float pz, float soft, float tiny, float& sx, float& sy,
                                            float& sz, float& cx, float& cy, float& cz) {
     float ax = 0.f, ay = 0.f, az = 0.f;
@@ -849,8 +858,8 @@ __device__ __forceinline__ void flush_list(const float4* lsrc, const uint2* lmas
         float dx[4], dy[4], dz[4], q[4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            src[g] = lsrc[e + g];
-            const uint2 m = lmask[e + g];
+            src[g] = lst[e + g].src;
+            const uint2 m = lst[e + g].mask;
             src[g].w = lane_gm(src[g].w, m.x, m.y, lane);
             q[g] = core::pair_arg<LAW>(src[g], px, py, pz, soft, tiny, dx[g], dy[g], dz[g]);
         }
@@ -866,8 +875,9 @@ __device__ __forceinline__ void flush_list(const float4* lsrc, const uint2* lmas
 // Workgroups are dealt to the accelerator's XCDs in turn (workgroup b to XCD
 // b mod 8), and each XCD has its own L2.  Neighbouring particle groups in
 // Morton order walk nearly the same cells, so the groups are renumbered to
-// give every XCD one contiguous eighth of the order: its L2 then holds the
-// child blocks its waves share.  A bijection on [0, nb).
+// give every XCD runs of 8 consecutive groups of the order (its L2 then holds
+// the child blocks those waves share), the runs dealt round the XCDs so that
+// dense and sparse regions spread over all of them.  A bijection on [0, nb).
 constexpr int kXcds = 8;
 
 __device__ __forceinline__ int xcd_block(int b, int nb) {
@@ -885,8 +895,7 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
                                                                 const int* __restrict__ node_last, TravArgs A) {
     __shared__ int stk_node[kTravWaves][kStackCap];
     __shared__ uint2 stk_mask[kTravWaves][kStackCap];
-    __shared__ float4 lst_src[kTravWaves][kListCap];
-    __shared__ uint2 lst_mask[kTravWaves][kListCap];
+    __shared__ ListEntry lst[kTravWaves][kListCap];
     __shared__ unsigned long long red[3][kTravWaves];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: list and stack addresses in SGPRs
@@ -903,10 +912,21 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
     int cnt = 0;
     int sp = 0;
 
+    // Append a source held in scalar registers (written by lane 0) ...
     auto append = [&](const float4 src, unsigned long long mask) {
         if (lane == 0) {
-            lst_src[wave][cnt] = src;
-            lst_mask[wave][cnt] = make_uint2(static_cast<unsigned>(mask), static_cast<unsigned>(mask >> 32));
+            lst[wave][cnt].src = src;
+            lst[wave][cnt].mask = make_uint2(static_cast<unsigned>(mask), static_cast<unsigned>(mask >> 32));
+        }
+        count += static_cast<unsigned long long>(__popcll(mask));
+        ++entries;
+        ++cnt;
+    };
+    // ... or child `slot` of the open block, already in lane `slot`'s registers.
+    auto append_child = [&](const float4& vsrc, int slot, unsigned long long mask) {
+        if (lane == slot) {
+            lst[wave][cnt].src = vsrc;
+            lst[wave][cnt].mask = make_uint2(static_cast<unsigned>(mask), static_cast<unsigned>(mask >> 32));
         }
         count += static_cast<unsigned long long>(__popcll(mask));
         ++entries;
@@ -915,13 +935,13 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
     auto flush = [&]() {
         if (lane == 0)
             while (cnt & 3) {
-                lst_src[wave][cnt] = make_float4(0.f, 0.f, 0.f, 0.f);
-                lst_mask[wave][cnt] = make_uint2(0u, 0u);
+                lst[wave][cnt].src = make_float4(0.f, 0.f, 0.f, 0.f);
+                lst[wave][cnt].mask = make_uint2(0u, 0u);
                 ++cnt;
             }
         cnt = (cnt + 3) & ~3;
         __builtin_amdgcn_wave_barrier();
-        flush_list<LAW>(lst_src[wave], lst_mask[wave], cnt, lane, me.x, me.y, me.z, soft, tiny, sx, sy, sz, cx, cy, cz);
+        flush_list<LAW>(lst[wave], cnt, lane, me.x, me.y, me.z, soft, tiny, sx, sy, sz, cx, cy, cz);
         __builtin_amdgcn_wave_barrier();
         cnt = 0;
     };
@@ -944,6 +964,15 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         const float r2 = (dx * dx + dy * dy) + dz * dz;
         const unsigned long long am = __ballot(r2 >= thr) & mask;
         if (am) append(c, am);
+        const unsigned long long om = mask & ~am;
+        if (om) push(code, om);
+    };
+    auto child_cell = [&](const float4 c, const float4& vsrc, int slot, float thr, int code,
+                          unsigned long long mask) {
+        const float dx = me.x - c.x, dy = me.y - c.y, dz = me.z - c.z;
+        const float r2 = (dx * dx + dy * dy) + dz * dz;
+        const unsigned long long am = __ballot(r2 >= thr) & mask;
+        if (am) append_child(vsrc, slot, am);
         const unsigned long long om = mask & ~am;
         if (om) push(code, om);
     };
@@ -983,6 +1012,7 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         // compiler from sinking each slot's loads into that slot's branch,
         // which would wait on memory once per cache line).
         const ChildBlock* B = blocks + code;
+        const float4 vsrc = B->src[lane & 7];  // lane s holds child s, for the list writes
         const int4 c0 = *reinterpret_cast<const int4*>(B->code);
         const int4 c1 = *reinterpret_cast<const int4*>(B->code + 4);
         int codes[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
@@ -1003,9 +1033,9 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
             if (c == kEmpty) continue;
             if (c <= -2) {
                 const unsigned long long m = without_self(mask, dec_leaf(c));
-                if (m) append(srcs[slot], m);
+                if (m) append_child(vsrc, slot, m);
             } else {
-                cell(srcs[slot], thrs[slot], c, mask);
+                child_cell(srcs[slot], vsrc, slot, thrs[slot], c, mask);
             }
         }
     }
