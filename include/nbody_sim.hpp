@@ -19,6 +19,7 @@
 #ifndef NBODY_SIM_HPP_
 #define NBODY_SIM_HPP_
 
+#include <algorithm>
 #include <cstddef>
 #include <memory>
 #include <stdexcept>
@@ -49,15 +50,17 @@ public:
     virtual ~INBodySim() = default;
     virtual void Init(std::vector<Particle>& particles) = 0;
     virtual void Update(float dt) = 0;
-    // The reference draws the octree's leaf cubes here (BarnesHut.cpp:98-101);
-    // the device simulator has nothing to draw, as INBodySim's default.
+    // The reference draws the octree's cubes here (BarnesHut.cpp:98-101); the
+    // device simulator draws nothing itself: DeviceNBodySim::Cells() gives the
+    // cubes to a renderer.
     virtual void RenderDebug(const float* /*view4x4*/, const float* /*proj4x4*/) {}
 };
 
 // The device simulator behind INBodySim: one nbody_sim handle.
 class DeviceNBodySim : public INBodySim {
 public:
-    explicit DeviceNBodySim(const nbody_config& cfg) : zero_forces_(cfg.method == NBODY_METHOD_DIRECT) {
+    explicit DeviceNBodySim(const nbody_config& cfg)
+        : zero_forces_(cfg.method == NBODY_METHOD_DIRECT), barnes_hut_(cfg.method == NBODY_METHOD_BARNES_HUT) {
         check("nbody_create", nbody_create(&cfg, &sim_));
     }
     ~DeviceNBodySim() override { nbody_destroy(sim_); }
@@ -80,6 +83,20 @@ public:
 
     nbody_sim* handle() { return sim_; }
 
+    // The octree cells of the last Barnes-Hut step (cube corner and side,
+    // centre of mass, mass, level, particle range): what the reference's
+    // Octree::RenderDebug draws (Octree.cpp:147-175).  Empty for the direct sum.
+    std::vector<nbody_bh_cell> Cells() {
+        std::vector<nbody_bh_cell> cells;
+        if (!barnes_hut_) return cells;
+        size_t count = 0;
+        check("nbody_bh_cells", nbody_bh_cells(sim_, nullptr, 0, &count));
+        cells.resize(count);
+        check("nbody_bh_cells", nbody_bh_cells(sim_, cells.data(), cells.size(), &count));
+        cells.resize(std::min(count, cells.size()));
+        return cells;
+    }
+
 private:
     static void check(const char* what, nbody_status st) {
         if (st != NBODY_OK) throw nbody_error(what, st);
@@ -87,6 +104,7 @@ private:
     nbody_sim* sim_ = nullptr;
     std::vector<Particle>* particles_ = nullptr;
     bool zero_forces_;
+    bool barnes_hut_;
 };
 
 // src/Sim/INBodySim.cpp:7-27.  Both brute-force simulators run the device

@@ -23,6 +23,9 @@ int main(int argc, char** argv) {
         auto sim = CreateNBodySim(nullptr, static_cast<ENBodySim>(kind));  // SimulationState.cpp:286
         sim->Init(particles);
         for (int s = 0; s < steps; ++s) sim->Update(0.02f / 60.0f);
+        // the octree the last Barnes-Hut step used, as a renderer would take it
+        const auto cells = static_cast<DeviceNBodySim*>(sim.get())->Cells();
+        std::fprintf(stderr, "cells %zu root %u\n", cells.size(), cells.empty() ? 0u : cells[0].count);
     } catch (const nbody_error& e) {
         std::fprintf(stderr, "%s\n", e.what());
         return 1;

@@ -111,6 +111,23 @@ def test_theta_zero_is_direct_sum(gpu):
     assert rel_err_rows(a_bh, a_d).max() < 1e-5  # two FP32 sums of the same pairs in different orders
 
 
+def test_plummer_law_against_fp64(gpu, oracle):
+    """Law P (G m r / (r^2 + eps^2)^(3/2)) through the octree: theta 0 is the
+    FP64 direct sum to FP32 rounding, theta 0.5 converges towards it."""
+    pu = gpu
+    n, soft = 4000, 1e-4
+    pos, vel, mass = pu.ic_plummer(n, 0x5EED000B)
+    ref = oracle.fp64_accel(1, pos, mass, 1.0, soft)
+    errs = {}
+    for theta in (0.0, 0.5):
+        cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_P, softening=soft, theta=theta)
+        with _bh(pu, cfg, pos, vel, mass) as sim:
+            errs[theta] = rel_err_rows(sim.accelerations(), ref)
+    print(f"law P: theta 0 max {errs[0.0].max():.2e}, theta 0.5 rms {np.sqrt(np.mean(errs[0.5] ** 2)):.2e}")
+    assert errs[0.0].max() < 1e-5
+    assert np.sqrt(np.mean(errs[0.5] ** 2)) < 2e-2
+
+
 def test_reference_ten_steps(gpu, oracle):
     pu = gpu
     n = 2048

@@ -38,6 +38,11 @@ def test_cpp_adapter_update_matches_oracle(gpu, oracle, tmp_path, kind):
     else:
         p_ref, v_ref = oracle.bh_step(pos, np.zeros((n, 3)), mass, dt, steps, 2.0, [-4000] * 3, 8000.0)
         assert np.any(vals[:, 6] != 0)  # Barnes-Hut leaves the last forces
+    cells = out.stderr.split()
+    if kind == 0:
+        assert cells[:2] == ["cells", "0"]
+    else:
+        assert int(cells[1]) > n // 8 and int(cells[3]) == n  # the root holds every particle
     assert rel_err_rows(vals[:, :3], p_ref).max() < 1e-4
 
 
