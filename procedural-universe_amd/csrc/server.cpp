@@ -34,6 +34,7 @@
 #include <deque>
 #include <map>
 #include <new>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -150,7 +151,15 @@ nbody_status admit(nbody_server* s, Request& r, bool* ok) {
             nbody_config c = s->cfg.sim;
             c.method = NBODY_METHOD_BARNES_HUT;
             NB_TRY(nbody_create(&c, &r.sim));
-            NB_TRY(nbody_set_graph(r.sim, 1));  // a few graph launches per tick instead of ~40 kernels a step
+            // one graph launch per step instead of about forty kernels
+            const nbody_status sg = nbody_set_graph(r.sim, 1);
+            if (sg != NBODY_OK) {
+                const std::string msg = nbody_last_error();  // nbody_destroy clears it
+                nbody_destroy(r.sim);
+                r.sim = nullptr;
+                set_error("%s", msg.c_str());
+                return sg;
+            }
             r.sim_cap = r.n;
             s->mem_used += f;
         }
