@@ -179,6 +179,20 @@ nbody_status nbody_read_soa(nbody_sim* sim, float* pos, float* vel, float* acc, 
  * nbody_ic_plummer, so nothing crosses PCIe. */
 nbody_status nbody_init_plummer(nbody_sim* sim, uint64_t seed, size_t n, double a, double total_mass);
 
+/* The reference's particle seeders (EParticleSeeder, src/Sim/IParticleSeeder.hpp:12-17). */
+typedef enum nbody_seeder {
+    NBODY_SEEDER_RANDOM = 0,      /* RandomSeeder.cpp:14-40: cube of half-width 500, radial 1e16 velocities    */
+    NBODY_SEEDER_GALAXY = 1,      /* GalaxySeeder.cpp:42-144: two spiral arms and a disc, random orientation  */
+    NBODY_SEEDER_STAR_SYSTEM = 2  /* StarSystemSeeder.cpp:17-55: a 1e30 star and bodies at 4-7 AU          */
+} nbody_seeder;
+
+/* Initialise with one of the reference's seeders' distributions, drawn in
+ * device memory from a counter-based stream keyed by (seed, particle): the
+ * same distributions (positions, velocities, masses; positions divided by
+ * scale), not the reference's MSVC random sequence.  Masses are copied back
+ * once for the FORCES read-back. */
+nbody_status nbody_init_seeded(nbody_sim* sim, int32_t seeder, uint64_t seed, size_t n, double scale);
+
 /* Evaluate accelerations at the current positions without stepping; acc
  * receives 3n floats.  Used for parity checks. */
 nbody_status nbody_compute_accelerations(nbody_sim* sim, float* acc, size_t n);

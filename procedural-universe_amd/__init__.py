@@ -65,6 +65,10 @@ STATUS_NAMES = {
 }
 
 METHOD_DIRECT = 1
+# nbody_seeder (EParticleSeeder)
+SEEDER_RANDOM = 0
+SEEDER_GALAXY = 1
+SEEDER_STAR_SYSTEM = 2
 METHOD_BARNES_HUT = 2
 LAW_R = 0
 LAW_P = 1
@@ -265,6 +269,7 @@ _SIGNATURES = {
     "nbody_synchronize": (ctypes.c_int, [_P]),
     "nbody_query": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32)]),
     "nbody_set_graph": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "nbody_init_seeded": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_double]),
     "nbody_read": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.c_uint32]),
     "nbody_read_soa": (ctypes.c_int, [_P, _F32P, _F32P, _F32P, ctypes.c_size_t]),
     "nbody_compute_accelerations": (ctypes.c_int, [_P, _F32P, ctypes.c_size_t]),
@@ -487,6 +492,13 @@ class NBodySim:
     def init_plummer(self, n: int, seed: int, a: float = 1.0, total_mass: float = 1.0):
         """Plummer sphere generated in device memory (same stream as ic_plummer)."""
         _check("nbody_init_plummer", self._lib.nbody_init_plummer(self._h, int(seed), int(n), float(a), float(total_mass)))
+        self._particles = None
+        self.n = n
+
+    def init_seeded(self, seeder: int, n: int, seed: int = 0, scale: float = 1.0):
+        """One of the reference's seeders (SEEDER_RANDOM, SEEDER_GALAXY,
+        SEEDER_STAR_SYSTEM) drawn in device memory."""
+        _check("nbody_init_seeded", self._lib.nbody_init_seeded(self._h, int(seeder), int(seed), int(n), float(scale)))
         self._particles = None
         self.n = n
 

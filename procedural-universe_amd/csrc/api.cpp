@@ -411,6 +411,36 @@ nbody_status step_graph(nbody_sim* s, float dt, int32_t nsteps) {
     return NBODY_OK;
 }
 
+nbody_status nbody_init_seeded(nbody_sim* s, int32_t seeder, uint64_t seed, size_t n, double scale) {
+    clear_error();
+    NB_CHECK(s, NBODY_ERR_INVALID_ARG, "sim is null");
+    NB_CHECK(seeder >= NBODY_SEEDER_RANDOM && seeder <= NBODY_SEEDER_STAR_SYSTEM, NBODY_ERR_INVALID_ARG,
+             "unknown seeder %d", seeder);
+    NB_CHECK(n >= 1 && n <= (size_t(1) << 30), NBODY_ERR_INVALID_ARG, "n = %zu out of range", n);
+    NB_CHECK(std::isfinite(scale) && scale > 0, NBODY_ERR_INVALID_ARG, "scale must be positive");
+    NB_TRY(prepare_state(s, n));
+    const size_t bytes = s->n_pad * sizeof(float4);
+    NB_HIP(hipMemsetAsync(s->pos[0], 0, bytes, s->stream));
+    NB_HIP(hipMemsetAsync(s->pos[1], 0, bytes, s->stream));
+    NB_HIP(hipMemsetAsync(s->vel, 0, bytes, s->stream));
+    NB_HIP(hipMemsetAsync(s->acc, 0, bytes, s->stream));
+    NB_HIP(hipMemsetAsync(s->mass_dev, 0, s->n_pad * sizeof(double), s->stream));
+    NB_HIP(hipMemsetAsync(s->gm_dev, 0, s->n_pad * sizeof(double), s->stream));
+    SeederTable T{};
+    ic_seeder_table(seed, &T);
+    NB_HIP(ic_device_seeded(seeder, seed, static_cast<int>(n), static_cast<float>(scale), s->cfg.G, T, s->pos[0],
+                            s->vel, s->mass_dev, s->gm_dev, s->pos[1], s->stream));
+    s->mass_host.resize(n);
+    NB_HIP(hipMemcpyAsync(s->mass_host.data(), s->mass_dev, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    NB_HIP(hipStreamSynchronize(s->stream));
+    s->max_gm = 0.0;
+    for (double m : s->mass_host) s->max_gm = std::max(s->max_gm, std::fabs(s->cfg.G * m));
+    s->initialised = true;
+    s->acc_valid = false;
+    if (s->bh) NB_TRY(bh_resize(s->bh, n, s));
+    return NBODY_OK;
+}
+
 nbody_status nbody_step(nbody_sim* s, float dt, int32_t nsteps) {
     clear_error();
     NB_CHECK(s, NBODY_ERR_INVALID_ARG, "sim is null");
