@@ -163,11 +163,13 @@ nbody_status nbody_query(nbody_sim* sim, int32_t* done);
 #define NBODY_FIELD_POSITION 1u
 #define NBODY_FIELD_VELOCITY 2u
 #define NBODY_FIELD_FORCES   4u
+#define NBODY_FIELD_MASS     8u
 
 /* Write the selected fields of the current state into the caller's records
  * (the reference's in-place update of `std::vector<Particle>`,
  * SimulationState.cpp:52-60).  FORCES is m * a of the last force evaluation
- * in FP64.  Other fields of the records are left untouched. */
+ * in FP64; MASS the FP64 masses (for states generated on the device).  Other
+ * fields of the records are left untouched. */
 nbody_status nbody_read(nbody_sim* sim, nbody_particle* particles, size_t n, uint32_t fields);
 
 /* Structure-of-arrays read: any pointer may be NULL.  acc is the last

@@ -77,6 +77,7 @@ ROOT_FIXED = 1
 FIELD_POSITION = 1
 FIELD_VELOCITY = 2
 FIELD_FORCES = 4
+FIELD_MASS = 8
 
 
 class ENBodySim(enum.IntEnum):

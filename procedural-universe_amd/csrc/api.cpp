@@ -527,7 +527,7 @@ nbody_status nbody_read(nbody_sim* s, nbody_particle* p, size_t n, uint32_t fiel
     clear_error();
     NB_CHECK(s, NBODY_ERR_INVALID_ARG, "sim is null");
     NB_CHECK(p || n == 0, NBODY_ERR_INVALID_ARG, "particles is null");
-    NB_CHECK((fields & ~7u) == 0, NBODY_ERR_INVALID_ARG, "unknown field bits 0x%x", fields);
+    NB_CHECK((fields & ~15u) == 0, NBODY_ERR_INVALID_ARG, "unknown field bits 0x%x", fields);
     std::vector<float> pos(3 * n), vel(3 * n), acc(3 * n);
     NB_TRY(nbody_read_soa(s, (fields & NBODY_FIELD_POSITION) ? pos.data() : nullptr,
                           (fields & NBODY_FIELD_VELOCITY) ? vel.data() : nullptr,
@@ -539,6 +539,7 @@ nbody_status nbody_read(nbody_sim* s, nbody_particle* p, size_t n, uint32_t fiel
             for (int c = 0; c < 3; ++c) p[i].velocity[c] = vel[3 * i + c];
         if (fields & NBODY_FIELD_FORCES)
             for (int c = 0; c < 3; ++c) p[i].forces[c] = s->mass_host[i] * static_cast<double>(acc[3 * i + c]);
+        if (fields & NBODY_FIELD_MASS) p[i].mass = s->mass_host[i];
     }
     return NBODY_OK;
 }

@@ -102,3 +102,16 @@ def test_seeded_galaxy_barnes_hut_parity(gpu, oracle):
     F, st = oracle.bh_forces(pos, mass, 2.0, [-4000.0] * 3, 8000.0, com_fp64=True)
     assert st["dropped"] == 0
     assert rel_err_rows(a, F / mass[:, None]).max() < 1e-5
+
+
+def test_read_back_masses(gpu):
+    """NBODY_FIELD_MASS returns the device-drawn masses (their sum is the FP64 diagnostic's)."""
+    pu = gpu
+    n = 3000
+    sim, _, _ = _seeded(pu, pu.SEEDER_GALAXY, n)
+    parts = np.zeros(n, dtype=pu.PARTICLE_DTYPE)
+    sim.read_particles(parts, pu.FIELD_MASS)
+    d = sim.diagnostics()
+    sim.close()
+    assert np.all(parts["position"] == 0)  # only the mass field was written
+    assert parts["mass"].sum() == pytest.approx(d["mass"], rel=1e-12)

@@ -68,3 +68,15 @@ def test_precompute_driver_writes_snapshot(gpu, tmp_path):
     r = subprocess.run([exe, "-f", out, "--steps", "2", "--out", out2], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert len(pu.read_snapshot(out2)) == 3000
+
+
+def test_benchmark_table(gpu):
+    """nbody_benchmark: RunBenchmark's table, one line per ENBodySim."""
+    exe = os.path.join(PKG, "nbody_benchmark")
+    r = subprocess.run([exe, "-p", "4096", "--seeder", "galaxy", "--frames", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = [l for l in r.stdout.splitlines() if not l.startswith("#")]
+    names = [l[:16].strip() for l in rows]
+    assert names == ["Brute Force CPU", "Brute Force GPU", "Barnes-Hut"]
+    assert all(float(l.split()[-2]) > 0 for l in rows)
