@@ -48,6 +48,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 #include <vector>
 
 #include "bh.hpp"
@@ -136,6 +137,11 @@ __device__ inline long long cell_index(float x, const RootParams& R, int a) {
 // holds is found by bisection over the ordered bit patterns of positive
 // floats, and the traversal's test becomes r2 >= thr, exactly the same
 // decision.  The side depends only on the level, so one threshold per level.
+// The traversal evaluates it as !(r2 < thr), which also holds for a NaN
+// distance (a NaN position then reaches the sums instead of being dropped).
+// When no finite r2 passes (theta <= 0) the threshold is +inf: only an
+// infinite r2 passes, where every pair and every monopole contributes
+// exactly 0 to the acceleration, as the reference's opened cell does.
 constexpr int kThrLevels = kBucketLevel + 1;
 
 __device__ float open_threshold(float side, float theta_lo) {
@@ -143,7 +149,7 @@ __device__ float open_threshold(float side, float theta_lo) {
         return __fdiv_rn(side, __fsqrt_rn(__uint_as_float(bits))) <= theta_lo;
     };
     const unsigned kInf = 0x7f800000u;
-    if (!approx(kInf)) return __uint_as_float(0x7fc00000u);  // NaN: r2 >= thr never holds
+    if (!approx(kInf)) return __uint_as_float(kInf);  // no finite r2 passes
     unsigned lo = 0u, hi = kInf;                               // !approx(lo), approx(hi)
     while (hi - lo > 1u) {
         const unsigned mid = lo + (hi - lo) / 2u;
@@ -501,71 +507,76 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(TreeArgs T) {
     if (i == 0) T.parent[0] = -1;
 }
 
-// Octree cells: kept[i] = 1 when node i starts a deeper cell than its parent;
-// a kept cell's child block starts with every slot empty.
+// Octree cells: kept[i] = 1 when node i starts a deeper cell than its parent.
 __global__ __launch_bounds__(256) void kept_kernel(const int* __restrict__ level, const int* __restrict__ parent,
-                                                   int m, int* __restrict__ kept, ChildBlock* __restrict__ blocks) {
+                                                   int m, int* __restrict__ kept) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= m) return;
-    const int k = (i == 0 || level[i] > level[parent[i]]) ? 1 : 0;
-    kept[i] = k;
-    if (k) {
-        int4* code = reinterpret_cast<int4*>(blocks[i].code);
-        code[0] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
-        code[1] = make_int4(kEmpty, kEmpty, kEmpty, kEmpty);
-    }
+    kept[i] = (i == 0 || level[i] > level[parent[i]]) ? 1 : 0;
 }
 
-// Every kept node (but the root) and every particle outside a bucket writes
-// itself into the slot of its octant in the nearest kept ancestor's child
-// block: (COM, G M), opening threshold and node for a cell, (position, G m)
-// for a particle.
-// Runs after the cells' centres of mass are final.
-struct LinkArgs {
-    const unsigned long long* keys;  // sorted
-    int n;
-    const int* parent;
-    const int* leaf_parent;
-    const int* first;
+// Every octree cell below the bucket level writes its own child block: its
+// children in key order (which is octant order) packed into the first slots,
+// the remaining codes kEmpty.  A child cell gets (COM, G M), its opening
+// threshold and its node; a particle (position, G m), threshold 0 (r2 >= 0
+// always holds) and its code.  The children are the first kept nodes and the
+// particles below the cell in the radix tree.  A node that is not kept has
+// its parent's level, so its common prefix ends inside the same 3-bit group
+// as the cell's; prefixes lengthen strictly from parent to child, so at most
+// two such nodes lie on any path below the cell and a descent of three
+// binary levels reaches every child.  Runs after the cells' centres of mass
+// are final; particles of a bucket are reached by its range instead.
+struct BlockArgs {
+    const int* left;
+    const int* right;
     const int* level;
     const int* kept;
     const float4* node_com;
     const float* thr;  // opening threshold per level
     const float4* pos_sorted;
+    int m;
     ChildBlock* blocks;
 };
 
-__global__ __launch_bounds__(256) void link_kernel(LinkArgs L) {
-    const int e = blockIdx.x * 256 + threadIdx.x;  // [0, n-1): internal nodes, [n-1, 2n-1): leaves
-    const int m = L.n - 1;
-    if (e >= m + L.n) return;
-    int q, code;
-    unsigned long long key;
-    float4 src;
-    float thr;
-    if (e < m) {
-        if (e == 0 || !L.kept[e]) return;
-        q = L.parent[e];
-        code = L.level[e] >= kBucketLevel ? (e | kBucketFlag) : e;
-        key = L.keys[L.first[e]];
-        src = L.node_com[e];
-        thr = L.thr[L.level[e]];
-    } else {
-        const int k = e - m;
-        q = L.leaf_parent[k];
-        code = enc_leaf(k);
-        key = L.keys[k];
-        src = L.pos_sorted[k];
-        thr = 0.f;
+__global__ __launch_bounds__(256) void block_kernel(BlockArgs A) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= A.m || !A.kept[i] || A.level[i] >= kBucketLevel) return;
+    ChildBlock& B = A.blocks[i];
+    int slot = 0;
+    auto terminal = [&](int c) { return c < 0 || A.kept[c] != 0; };
+    auto emit = [&](int c) {
+        if (slot >= 8) return;  // cannot happen (at most 2^3 children); keeps a bad tree in bounds
+        if (c < 0) {
+            B.src[slot] = A.pos_sorted[dec_leaf(c)];
+            B.thr[slot] = 0.f;
+            B.code[slot] = c;
+        } else {
+            const int lv = A.level[c];
+            B.src[slot] = A.node_com[c];
+            B.thr[slot] = A.thr[lv];
+            B.code[slot] = lv >= kBucketLevel ? (c | kBucketFlag) : c;
+        }
+        ++slot;
+    };
+    const int top[2] = {A.left[i], A.right[i]};
+    for (int a = 0; a < 2; ++a) {
+        const int c1 = top[a];
+        if (terminal(c1)) {
+            emit(c1);
+            continue;
+        }
+        const int mid[2] = {A.left[c1], A.right[c1]};
+        for (int b = 0; b < 2; ++b) {
+            const int c2 = mid[b];
+            if (terminal(c2)) {
+                emit(c2);
+                continue;
+            }
+            emit(A.left[c2]);
+            emit(A.right[c2]);
+        }
     }
-    while (!L.kept[q]) q = L.parent[q];
-    const int lv = L.level[q];
-    if (lv >= kBucketLevel) return;  // particles of a bucket are reached by its range
-    const int slot = static_cast<int>((key >> (60 - 3 * lv)) & 7ULL);
-    ChildBlock& B = L.blocks[q];
-    B.src[slot] = src;
-    B.thr[slot] = thr;
-    B.code[slot] = code;
+    for (int s = slot; s < 8; ++s) B.code[s] = kEmpty;
 }
 
 // ---- 5. FP64 mass and centre of mass ------------------------------------------------
@@ -848,6 +859,12 @@ __device__ __forceinline__ float lane_gm(float gm, unsigned lo, unsigned hi, int
     return __int_as_float(__float_as_int(gm) & __builtin_amdgcn_sbfe(static_cast<int>(w), lane & 31, 1));
 }
 
+// v = x in lane L only (a register write; the other lanes keep v).
+template <int L>
+__device__ __forceinline__ void write_lane(unsigned& v, unsigned x) {
+    asm("v_writelane_b32 %0, %1, %2" : "+v"(v) : "s"(x), "n"(L));
+}
+
 template <int LAW>
 __device__ __forceinline__ void flush_list(const ListEntry* lst, int cnt, int lane, float px,
                                            float py, float pz, float soft, float tiny, float& sx, float& sy,
@@ -911,70 +928,40 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
     unsigned long long count = 0, entries = 0, opened = 0;
     int cnt = 0;
     int sp = 0;
+    bool overflowed = false;
+    // interactions of the entries this lane wrote (lanes 0-7), folded into a
+    // 64-bit per-lane total at every flush
+    unsigned lane_count = 0;
+    unsigned long long lane_total = 0;
+    // child s's lane masks (approximating, opening) in lane s during a pop;
+    // lanes beyond the block's children hold stale values and are ignored
+    unsigned am_lo = 0u, am_hi = 0u, om_lo = 0u, om_hi = 0u;
+    // a lane's own particle as a child code
+    const int self_code = valid ? enc_leaf(t) : kEmpty;
 
-    // Append a source held in scalar registers (written by lane 0) ...
+    // Append a source held in scalar registers (written by lane 0).
     auto append = [&](const float4 src, unsigned long long mask) {
         if (lane == 0) {
             lst[wave][cnt].src = src;
             lst[wave][cnt].mask = make_uint2(static_cast<unsigned>(mask), static_cast<unsigned>(mask >> 32));
         }
         count += static_cast<unsigned long long>(__popcll(mask));
-        ++entries;
-        ++cnt;
-    };
-    // ... or child `slot` of the open block, already in lane `slot`'s registers.
-    auto append_child = [&](const float4& vsrc, int slot, unsigned long long mask) {
-        if (lane == slot) {
-            lst[wave][cnt].src = vsrc;
-            lst[wave][cnt].mask = make_uint2(static_cast<unsigned>(mask), static_cast<unsigned>(mask >> 32));
-        }
-        count += static_cast<unsigned long long>(__popcll(mask));
-        ++entries;
         ++cnt;
     };
     auto flush = [&]() {
-        if (lane == 0)
-            while (cnt & 3) {
-                lst[wave][cnt].src = make_float4(0.f, 0.f, 0.f, 0.f);
-                lst[wave][cnt].mask = make_uint2(0u, 0u);
-                ++cnt;
-            }
-        cnt = (cnt + 3) & ~3;
+        entries += static_cast<unsigned long long>(cnt);
+        const int pad = (4 - (cnt & 3)) & 3;
+        if (lane < pad) {
+            lst[wave][cnt + lane].src = make_float4(0.f, 0.f, 0.f, 0.f);
+            lst[wave][cnt + lane].mask = make_uint2(0u, 0u);
+        }
+        const int total = __builtin_amdgcn_readfirstlane(cnt + pad);
         __builtin_amdgcn_wave_barrier();
-        flush_list<LAW>(lst[wave], cnt, lane, me.x, me.y, me.z, soft, tiny, sx, sy, sz, cx, cy, cz);
+        flush_list<LAW>(lst[wave], total, lane, me.x, me.y, me.z, soft, tiny, sx, sy, sz, cx, cy, cz);
         __builtin_amdgcn_wave_barrier();
         cnt = 0;
-    };
-    auto push = [&](int code, unsigned long long mask) {
-        if (sp < kStackCap) {
-            if (lane == 0) {
-                stk_node[wave][sp] = code;
-                stk_mask[wave][sp] = make_uint2(static_cast<unsigned>(mask), static_cast<unsigned>(mask >> 32));
-            }
-            ++sp;
-        } else if (lane == 0) {
-            atomicOr(A.overflow, 1u);
-        }
-    };
-    // A cell met by the lanes of `mask`: each approximates it (the reference's
-    // side / |p - COM| < theta as r2 >= thr, see open_threshold_kernel) or
-    // opens it.  r2 in the reference's order, (dx dx + dy dy) + dz dz, unfused.
-    auto cell = [&](const float4 c, float thr, int code, unsigned long long mask) {
-        const float dx = me.x - c.x, dy = me.y - c.y, dz = me.z - c.z;
-        const float r2 = (dx * dx + dy * dy) + dz * dz;
-        const unsigned long long am = __ballot(r2 >= thr) & mask;
-        if (am) append(c, am);
-        const unsigned long long om = mask & ~am;
-        if (om) push(code, om);
-    };
-    auto child_cell = [&](const float4 c, const float4& vsrc, int slot, float thr, int code,
-                          unsigned long long mask) {
-        const float dx = me.x - c.x, dy = me.y - c.y, dz = me.z - c.z;
-        const float r2 = (dx * dx + dy * dy) + dz * dz;
-        const unsigned long long am = __ballot(r2 >= thr) & mask;
-        if (am) append_child(vsrc, slot, am);
-        const unsigned long long om = mask & ~am;
-        if (om) push(code, om);
+        lane_total += lane_count;
+        lane_count = 0;
     };
     // the lanes of a particle's own position never interact with it
     auto without_self = [&](unsigned long long m, int q) {
@@ -984,8 +971,23 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
 
     const unsigned long long all = __ballot(valid);
     if (all) {
+        // the root: approximated by the lanes beyond its threshold (the
+        // reference's side / |p - COM| < theta as r2 >= thr, see
+        // open_threshold), opened by the others
         const int root_level = A.node_level[0];
-        cell(A.node_com[0], A.thr[root_level], root_level >= kBucketLevel ? kBucketFlag : 0, all);
+        const float4 c = A.node_com[0];
+        const float dx = me.x - c.x, dy = me.y - c.y, dz = me.z - c.z;
+        const float r2 = (dx * dx + dy * dy) + dz * dz;
+        const unsigned long long am = __ballot(!(r2 < A.thr[root_level])) & all;
+        if (am) append(c, am);
+        const unsigned long long om = all & ~am;
+        if (om) {
+            if (lane == 0) {
+                stk_node[wave][0] = root_level >= kBucketLevel ? kBucketFlag : 0;
+                stk_mask[wave][0] = make_uint2(static_cast<unsigned>(om), static_cast<unsigned>(om >> 32));
+            }
+            sp = 1;
+        }
     }
     while (sp > 0) {
         --sp;
@@ -1010,9 +1012,11 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         if (cnt > kListCap - 8) flush();
         // The whole block in scalar registers at once (the empty asm keeps the
         // compiler from sinking each slot's loads into that slot's branch,
-        // which would wait on memory once per cache line).
+        // which would wait on memory once per cache line), and child s's
+        // source and code in lane s for the list and stack writes.
         const ChildBlock* B = blocks + code;
-        const float4 vsrc = B->src[lane & 7];  // lane s holds child s, for the list writes
+        const float4 vsrc = B->src[lane & 7];
+        const int vcode = B->code[lane & 7];
         const int4 c0 = *reinterpret_cast<const int4*>(B->code);
         const int4 c1 = *reinterpret_cast<const int4*>(B->code + 4);
         int codes[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
@@ -1027,19 +1031,79 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         for (int slot = 0; slot < 8; ++slot)
             asm volatile("" ::"s"(srcs[slot].x), "s"(srcs[slot].y), "s"(srcs[slot].z), "s"(srcs[slot].w),
                          "s"(thrs[slot]), "s"(codes[slot]));
-#pragma unroll
-        for (int slot = 0; slot < 8; ++slot) {
-            const int c = codes[slot];
-            if (c == kEmpty) continue;
-            if (c <= -2) {
-                const unsigned long long m = without_self(mask, dec_leaf(c));
-                if (m) append_child(vsrc, slot, m);
-            } else {
-                child_cell(srcs[slot], vsrc, slot, thrs[slot], c, mask);
+        // Decide every child without branches: a lane approximates child s
+        // (a cell's monopole, or a particle: threshold 0) when !(r2 < thr),
+        // except its own particle; it opens a cell when r2 < thr (a particle
+        // never: r2 < 0 is false).  The children are packed, the first empty
+        // code ends them.  Child s's two lane masks go to lane s.
+        auto decide = [&](auto S) {
+            constexpr int slot = decltype(S)::value;
+            const float dx = me.x - srcs[slot].x, dy = me.y - srcs[slot].y, dz = me.z - srcs[slot].z;
+            const float r2 = (dx * dx + dy * dy) + dz * dz;
+            const unsigned long long ge = __ballot(!(r2 < thrs[slot]));
+            const unsigned long long am = ge & __ballot(self_code != codes[slot]) & mask;
+            const unsigned long long om = mask & ~ge;
+            write_lane<slot>(am_lo, static_cast<unsigned>(am));
+            write_lane<slot>(am_hi, static_cast<unsigned>(am >> 32));
+            write_lane<slot>(om_lo, static_cast<unsigned>(om));
+            write_lane<slot>(om_hi, static_cast<unsigned>(om >> 32));
+        };
+        using std::integral_constant;
+        // a cell has at least two children
+        decide(integral_constant<int, 0>{});
+        decide(integral_constant<int, 1>{});
+        if (codes[2] != kEmpty) {
+            decide(integral_constant<int, 2>{});
+            if (codes[3] != kEmpty) {
+                decide(integral_constant<int, 3>{});
+                if (codes[4] != kEmpty) {
+                    decide(integral_constant<int, 4>{});
+                    if (codes[5] != kEmpty) {
+                        decide(integral_constant<int, 5>{});
+                        if (codes[6] != kEmpty) {
+                            decide(integral_constant<int, 6>{});
+                            if (codes[7] != kEmpty) decide(integral_constant<int, 7>{});
+                        }
+                    }
+                }
+            }
+        }
+        // List entries, in child order, written by the children's lanes.
+        const bool child = lane < 8 && vcode != kEmpty;
+        const bool wl = child && ((static_cast<unsigned long long>(am_hi) << 32) | am_lo) != 0ull;
+        const unsigned long long has_l = __ballot(wl);
+        if (has_l) {
+            if (wl) {
+                const int at = static_cast<int>(__builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_l), cnt));
+                lst[wave][at].src = vsrc;
+                lst[wave][at].mask = make_uint2(am_lo, am_hi);
+                lane_count += __popc(am_lo) + __popc(am_hi);
+            }
+            cnt += __popcll(has_l);
+        }
+        // Opened cells onto the stack, in child order (the last is popped first).
+        const bool ws = lane < 8 && vcode >= 0 && ((static_cast<unsigned long long>(om_hi) << 32) | om_lo) != 0ull;
+        const unsigned long long has_s = __ballot(ws);
+        if (has_s) {
+            const int base = sp;
+            sp = __builtin_amdgcn_readfirstlane(base + __popcll(has_s));
+            if (ws) {
+                const int at = static_cast<int>(__builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_s), base));
+                if (at < kStackCap) {
+                    stk_node[wave][at] = vcode;
+                    stk_mask[wave][at] = make_uint2(om_lo, om_hi);
+                }
+            }
+            if (sp > kStackCap) {
+                sp = kStackCap;
+                overflowed = true;
             }
         }
     }
+    if (overflowed && lane == 0) atomicOr(A.overflow, 1u);
     if (cnt) flush();
+    for (int off = 32; off > 0; off >>= 1) lane_total += __shfl_xor(lane_total, off);
+    count += lane_total;
 
     if (valid) {
         const unsigned p = A.order[t];
@@ -1315,8 +1379,7 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     const int m = n - 1;
     TreeArgs T{keys, n, b->left, b->right, b->parent, b->leaf_parent, b->first, b->last, b->level};
     hipLaunchKernelGGL(radix_tree_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, T);
-    hipLaunchKernelGGL(kept_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->level, b->parent, m, b->kept,
-                       b->blocks);
+    hipLaunchKernelGGL(kept_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->level, b->parent, m, b->kept);
     const int mchunks = blocks_for(n + 1, kMassChunk);
     MassArgs M{b->pos_sorted, order, s->mass_dev, n, b->prefix, b->mass_chunk, mchunks};
     hipLaunchKernelGGL(mass_reduce_kernel, dim3(mchunks), dim3(kMassThreads), 0, st, M);
@@ -1324,10 +1387,9 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     hipLaunchKernelGGL(mass_prefix_kernel, dim3(mchunks), dim3(kMassThreads), 0, st, M);
     hipLaunchKernelGGL(cell_com_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->prefix, n, b->first, b->last,
                        b->kept, m, s->cfg.G, b->com);
-    LinkArgs L{keys, n, b->parent, b->leaf_parent, b->first, b->level, b->kept, b->com, b->thr, b->pos_sorted,
-               b->blocks};
+    BlockArgs L{b->left, b->right, b->level, b->kept, b->com, b->thr, b->pos_sorted, m, b->blocks};
     b->built = true;
-    hipLaunchKernelGGL(link_kernel, dim3(blocks_for(m + n, 256)), dim3(256), 0, st, L);
+    hipLaunchKernelGGL(block_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, L);
     NB_HIP(hipGetLastError());
     return NBODY_OK;
 }
