@@ -859,10 +859,15 @@ __device__ __forceinline__ float lane_gm(float gm, unsigned lo, unsigned hi, int
     return __int_as_float(__float_as_int(gm) & __builtin_amdgcn_sbfe(static_cast<int>(w), lane & 31, 1));
 }
 
-// v = x in lane L only (a register write; the other lanes keep v).
+// (lo, hi) = x in lane L only (register writes; the other lanes keep lo and
+// hi).  x is usually a compare result: a VALU write of VCC read by a VALU
+// instruction as a scalar operand needs one wait state, of another SGPR pair
+// two, and the compiler does not look inside the asm, so the asm waits two.
 template <int L>
-__device__ __forceinline__ void write_lane(unsigned& v, unsigned x) {
-    asm("v_writelane_b32 %0, %1, %2" : "+v"(v) : "s"(x), "n"(L));
+__device__ __forceinline__ void write_lane64(unsigned& lo, unsigned& hi, unsigned long long x) {
+    asm("s_nop 1\n\tv_writelane_b32 %0, %2, %4\n\tv_writelane_b32 %1, %3, %4"
+        : "+v"(lo), "+v"(hi)
+        : "s"(static_cast<unsigned>(x)), "s"(static_cast<unsigned>(x >> 32)), "n"(L));
 }
 
 template <int LAW>
@@ -933,11 +938,10 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
     // 64-bit per-lane total at every flush
     unsigned lane_count = 0;
     unsigned long long lane_total = 0;
-    // child s's lane masks (approximating, opening) in lane s during a pop;
-    // lanes beyond the block's children hold stale values and are ignored
-    unsigned am_lo = 0u, am_hi = 0u, om_lo = 0u, om_hi = 0u;
-    // a lane's own particle as a child code
-    const int self_code = valid ? enc_leaf(t) : kEmpty;
+    // during a pop, child cell s's mask of the lanes at or beyond its opening
+    // threshold, in lane s; lanes of particle children and lanes beyond the
+    // block's children hold stale values and are not used
+    unsigned ge_lo = 0u, ge_hi = 0u;
 
     // Append a source held in scalar registers (written by lane 0).
     auto append = [&](const float4 src, unsigned long long mask) {
@@ -969,7 +973,7 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         return (l >= 0 && l < 64) ? (m & ~(1ULL << l)) : m;
     };
 
-    const unsigned long long all = __ballot(valid);
+    const unsigned long long all = __builtin_amdgcn_ballot_w64(valid);
     if (all) {
         // the root: approximated by the lanes beyond its threshold (the
         // reference's side / |p - COM| < theta as r2 >= thr, see
@@ -978,7 +982,7 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         const float4 c = A.node_com[0];
         const float dx = me.x - c.x, dy = me.y - c.y, dz = me.z - c.z;
         const float r2 = (dx * dx + dy * dy) + dz * dz;
-        const unsigned long long am = __ballot(!(r2 < A.thr[root_level])) & all;
+        const unsigned long long am = __builtin_amdgcn_ballot_w64(!(r2 < A.thr[root_level])) & all;
         if (am) append(c, am);
         const unsigned long long om = all & ~am;
         if (om) {
@@ -1031,25 +1035,21 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         for (int slot = 0; slot < 8; ++slot)
             asm volatile("" ::"s"(srcs[slot].x), "s"(srcs[slot].y), "s"(srcs[slot].z), "s"(srcs[slot].w),
                          "s"(thrs[slot]), "s"(codes[slot]));
-        // Decide every child without branches: a lane approximates child s
-        // (a cell's monopole, or a particle: threshold 0) when !(r2 < thr),
-        // except its own particle; it opens a cell when r2 < thr (a particle
-        // never: r2 < 0 is false).  The children are packed, the first empty
-        // code ends them.  Child s's two lane masks go to lane s.
+        // A child particle needs no test: every lane of the mask but its own
+        // takes it.  A child cell is approximated by the lanes with
+        // !(r2 < thr) (the reference's side / |p - COM| < theta, see
+        // open_threshold) and opened by the others; that lane mask goes to
+        // lane s.  The children are packed, the first empty code ends them;
+        // a cell has at least two.
         auto decide = [&](auto S) {
             constexpr int slot = decltype(S)::value;
+            if (codes[slot] < 0) return;
             const float dx = me.x - srcs[slot].x, dy = me.y - srcs[slot].y, dz = me.z - srcs[slot].z;
             const float r2 = (dx * dx + dy * dy) + dz * dz;
-            const unsigned long long ge = __ballot(!(r2 < thrs[slot]));
-            const unsigned long long am = ge & __ballot(self_code != codes[slot]) & mask;
-            const unsigned long long om = mask & ~ge;
-            write_lane<slot>(am_lo, static_cast<unsigned>(am));
-            write_lane<slot>(am_hi, static_cast<unsigned>(am >> 32));
-            write_lane<slot>(om_lo, static_cast<unsigned>(om));
-            write_lane<slot>(om_hi, static_cast<unsigned>(om >> 32));
+            const unsigned long long ge = __builtin_amdgcn_ballot_w64(!(r2 < thrs[slot]));
+            write_lane64<slot>(ge_lo, ge_hi, ge);
         };
         using std::integral_constant;
-        // a cell has at least two children
         decide(integral_constant<int, 0>{});
         decide(integral_constant<int, 1>{});
         if (codes[2] != kEmpty) {
@@ -1068,22 +1068,31 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
                 }
             }
         }
-        // List entries, in child order, written by the children's lanes.
+        // Lane s forms child s's two masks: approximating lanes am (for a
+        // particle every lane of the mask but the particle's own) and
+        // opening lanes om (none for a particle).
         const bool child = lane < 8 && vcode != kEmpty;
-        const bool wl = child && ((static_cast<unsigned long long>(am_hi) << 32) | am_lo) != 0ull;
-        const unsigned long long has_l = __ballot(wl);
+        const bool leaf = vcode < 0;
+        const unsigned own = static_cast<unsigned>(dec_leaf(vcode) - wave_base);
+        const unsigned long long self = (leaf && own < 64u) ? (1ull << own) : 0ull;
+        const unsigned long long ge = leaf ? ~0ull : ((static_cast<unsigned long long>(ge_hi) << 32) | ge_lo);
+        const unsigned long long am = mask & ge & ~self;
+        const unsigned long long om = mask & ~ge;
+        // List entries, in child order, written by the children's lanes.
+        const bool wl = child && am != 0ull;
+        const unsigned long long has_l = __builtin_amdgcn_ballot_w64(wl);
         if (has_l) {
             if (wl) {
                 const int at = static_cast<int>(__builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_l), cnt));
                 lst[wave][at].src = vsrc;
-                lst[wave][at].mask = make_uint2(am_lo, am_hi);
-                lane_count += __popc(am_lo) + __popc(am_hi);
+                lst[wave][at].mask = make_uint2(static_cast<unsigned>(am), static_cast<unsigned>(am >> 32));
+                lane_count += __popcll(am);
             }
             cnt += __popcll(has_l);
         }
         // Opened cells onto the stack, in child order (the last is popped first).
-        const bool ws = lane < 8 && vcode >= 0 && ((static_cast<unsigned long long>(om_hi) << 32) | om_lo) != 0ull;
-        const unsigned long long has_s = __ballot(ws);
+        const bool ws = child && om != 0ull;  // a cell: a particle's ge is all lanes
+        const unsigned long long has_s = __builtin_amdgcn_ballot_w64(ws);
         if (has_s) {
             const int base = sp;
             sp = __builtin_amdgcn_readfirstlane(base + __popcll(has_s));
@@ -1091,7 +1100,7 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
                 const int at = static_cast<int>(__builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_s), base));
                 if (at < kStackCap) {
                     stk_node[wave][at] = vcode;
-                    stk_mask[wave][at] = make_uint2(om_lo, om_hi);
+                    stk_mask[wave][at] = make_uint2(static_cast<unsigned>(om), static_cast<unsigned>(om >> 32));
                 }
             }
             if (sp > kStackCap) {
