@@ -853,9 +853,9 @@ struct ListEntry {
 };
 
 // G*m where the lane's bit of the entry mask is set, else 0 (the bit
-// sign-extended to a full-width and-mask).
-__device__ __forceinline__ float lane_gm(float gm, unsigned lo, unsigned hi, int lane) {
-    const unsigned w = lane < 32 ? lo : hi;
+// sign-extended to a full-width and-mask); w is the mask word holding the
+// lane's bit (lanes 0-31 read the low word, lanes 32-63 the high one).
+__device__ __forceinline__ float lane_gm(float gm, unsigned w, int lane) {
     return __int_as_float(__float_as_int(gm) & __builtin_amdgcn_sbfe(static_cast<int>(w), lane & 31, 1));
 }
 
@@ -875,14 +875,15 @@ __device__ __forceinline__ void flush_list(const ListEntry* lst, int cnt, int la
                                            float py, float pz, float soft, float tiny, float& sx, float& sy,
                                            float& sz, float& cx, float& cy, float& cz) {
     float ax = 0.f, ay = 0.f, az = 0.f;
+    // the lane's word of every entry's mask (entries are 8 words apart)
+    const unsigned* words = reinterpret_cast<const unsigned*>(&lst[0].mask) + (lane >> 5);
     for (int e = 0; e < cnt; e += 4) {
         float4 src[4];
         float dx[4], dy[4], dz[4], q[4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             src[g] = lst[e + g].src;
-            const uint2 m = lst[e + g].mask;
-            src[g].w = lane_gm(src[g].w, m.x, m.y, lane);
+            src[g].w = lane_gm(src[g].w, words[(e + g) * 8], lane);
             q[g] = core::pair_arg<LAW>(src[g], px, py, pz, soft, tiny, dx[g], dy[g], dz[g]);
         }
         core::rsq4(q[0], q[1], q[2], q[3]);
@@ -1072,6 +1073,7 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         // particle every lane of the mask but the particle's own) and
         // opening lanes om (none for a particle).
         const bool child = lane < 8 && vcode != kEmpty;
+        const unsigned long long children = __builtin_amdgcn_ballot_w64(vcode != kEmpty) & 0xffull;
         const bool leaf = vcode < 0;
         const unsigned own = static_cast<unsigned>(dec_leaf(vcode) - wave_base);
         const unsigned long long self = (leaf && own < 64u) ? (1ull << own) : 0ull;
@@ -1080,22 +1082,27 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         const unsigned long long om = mask & ~ge;
         // List entries, in child order, written by the children's lanes.
         const bool wl = child && am != 0ull;
-        const unsigned long long has_l = __builtin_amdgcn_ballot_w64(wl);
-        if (has_l) {
-            if (wl) {
-                const int at = static_cast<int>(__builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_l), cnt));
-                lst[wave][at].src = vsrc;
-                lst[wave][at].mask = make_uint2(static_cast<unsigned>(am), static_cast<unsigned>(am >> 32));
-                lane_count += __popcll(am);
-            }
-            cnt += __popcll(has_l);
+        const unsigned long long has_l = __builtin_amdgcn_ballot_w64(am != 0ull) & children;
+        if (wl) {
+            const int at = static_cast<int>(__builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_l), cnt));
+            lst[wave][at].src = vsrc;
+            lst[wave][at].mask = make_uint2(static_cast<unsigned>(am), static_cast<unsigned>(am >> 32));
+            lane_count += __popcll(am);
         }
-        // Opened cells onto the stack, in child order (the last is popped first).
+        cnt += __popcll(has_l);
+        // Opened cells onto the stack, in child order (the last is popped
+        // first); at most eight, so the room is checked once.
         const bool ws = child && om != 0ull;  // a cell: a particle's ge is all lanes
-        const unsigned long long has_s = __builtin_amdgcn_ballot_w64(ws);
-        if (has_s) {
-            const int base = sp;
-            sp = __builtin_amdgcn_readfirstlane(base + __popcll(has_s));
+        const unsigned long long has_s = __builtin_amdgcn_ballot_w64(om != 0ull) & children;
+        const int base = sp;
+        if (base <= kStackCap - 8) {
+            if (ws) {
+                const int at = static_cast<int>(__builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_s), base));
+                stk_node[wave][at] = vcode;
+                stk_mask[wave][at] = make_uint2(static_cast<unsigned>(om), static_cast<unsigned>(om >> 32));
+            }
+            sp = base + __popcll(has_s);
+        } else {
             if (ws) {
                 const int at = static_cast<int>(__builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_s), base));
                 if (at < kStackCap) {
@@ -1103,6 +1110,7 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
                     stk_mask[wave][at] = make_uint2(static_cast<unsigned>(om), static_cast<unsigned>(om >> 32));
                 }
             }
+            sp = base + __popcll(has_s);
             if (sp > kStackCap) {
                 sp = kStackCap;
                 overflowed = true;
