@@ -461,16 +461,20 @@ __device__ inline int delta(const unsigned long long* __restrict__ keys, int n, 
 // the top bit is always 0): the number of whole 3-bit groups in common.
 __device__ inline int level_of(int d) { return d >= 64 ? kBucketLevel : (d - 1) / 3; }
 
+// A node's level and whether it is an octree cell ("kept": it starts a
+// deeper cell than its parent, or it is the root), in one word.
+constexpr int kLevelMask = 0xff;
+constexpr int kKeptBit = 1 << 8;
+__host__ __device__ inline int info_level(int info) { return info & kLevelMask; }
+__host__ __device__ inline bool info_kept(int info) { return (info & kKeptBit) != 0; }
+
 struct TreeArgs {
     const unsigned long long* keys;  // sorted
     int n;
-    int* left;         // [n-1] encoded child
-    int* right;        // [n-1] encoded child
-    int* parent;       // [n-1] parent of internal node (-1 at the root)
-    int* leaf_parent;  // [n]
-    int* first;        // [n-1]
-    int* last;         // [n-1]
-    int* level;        // [n-1]
+    int2* child;  // [n-1] encoded (left, right) children
+    int* first;   // [n-1]
+    int* last;    // [n-1]
+    int* info;    // [n-1] level | kKeptBit
 };
 
 __global__ __launch_bounds__(256) void radix_tree_kernel(TreeArgs T) {
@@ -497,22 +501,21 @@ __global__ __launch_bounds__(256) void radix_tree_kernel(TreeArgs T) {
     const int lo = min(i, j), hi = max(i, j);
     const int lc = (lo == gamma) ? enc_leaf(gamma) : gamma;
     const int rc = (hi == gamma + 1) ? enc_leaf(gamma + 1) : gamma + 1;
-    T.left[i] = lc;
-    T.right[i] = rc;
+    const int lv = level_of(dnode);
+    T.child[i] = make_int2(lc, rc);
     T.first[i] = lo;
     T.last[i] = hi;
-    T.level[i] = level_of(dnode);
-    if (lc >= 0) T.parent[lc] = i; else T.leaf_parent[dec_leaf(lc)] = i;
-    if (rc >= 0) T.parent[rc] = i; else T.leaf_parent[dec_leaf(rc)] = i;
-    if (i == 0) T.parent[0] = -1;
-}
-
-// Octree cells: kept[i] = 1 when node i starts a deeper cell than its parent.
-__global__ __launch_bounds__(256) void kept_kernel(const int* __restrict__ level, const int* __restrict__ parent,
-                                                   int m, int* __restrict__ kept) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= m) return;
-    kept[i] = (i == 0 || level[i] > level[parent[i]]) ? 1 : 0;
+    // The node writes its internal children's words: a child's common prefix
+    // is that of its range's end points, the same delta its own thread finds.
+    if (lc >= 0) {
+        const int lvc = level_of(delta(keys, n, lo, gamma));
+        T.info[lc] = lvc | (lvc > lv ? kKeptBit : 0);
+    }
+    if (rc >= 0) {
+        const int lvc = level_of(delta(keys, n, gamma + 1, hi));
+        T.info[rc] = lvc | (lvc > lv ? kKeptBit : 0);
+    }
+    if (i == 0) T.info[0] = lv | kKeptBit;
 }
 
 // Every octree cell below the bucket level writes its own child block: its
@@ -527,10 +530,8 @@ __global__ __launch_bounds__(256) void kept_kernel(const int* __restrict__ level
 // binary levels reaches every child.  Runs after the cells' centres of mass
 // are final; particles of a bucket are reached by its range instead.
 struct BlockArgs {
-    const int* left;
-    const int* right;
-    const int* level;
-    const int* kept;
+    const int2* child;
+    const int* info;
     const float4* node_com;
     const float* thr;  // opening threshold per level
     const float4* pos_sorted;
@@ -540,40 +541,45 @@ struct BlockArgs {
 
 __global__ __launch_bounds__(256) void block_kernel(BlockArgs A) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= A.m || !A.kept[i] || A.level[i] >= kBucketLevel) return;
+    if (i >= A.m) return;
+    const int self = A.info[i];
+    if (!info_kept(self) || info_level(self) >= kBucketLevel) return;
     ChildBlock& B = A.blocks[i];
     int slot = 0;
-    auto terminal = [&](int c) { return c < 0 || A.kept[c] != 0; };
-    auto emit = [&](int c) {
+    // a child particle (c < 0), or a child cell with its word
+    auto emit = [&](int c, int info) {
         if (slot >= 8) return;  // cannot happen (at most 2^3 children); keeps a bad tree in bounds
         if (c < 0) {
             B.src[slot] = A.pos_sorted[dec_leaf(c)];
             B.thr[slot] = 0.f;
             B.code[slot] = c;
         } else {
-            const int lv = A.level[c];
+            const int lv = info_level(info);
             B.src[slot] = A.node_com[c];
             B.thr[slot] = A.thr[lv];
             B.code[slot] = lv >= kBucketLevel ? (c | kBucketFlag) : c;
         }
         ++slot;
     };
-    const int top[2] = {A.left[i], A.right[i]};
+    const int2 top = A.child[i];
     for (int a = 0; a < 2; ++a) {
-        const int c1 = top[a];
-        if (terminal(c1)) {
-            emit(c1);
+        const int c1 = a ? top.y : top.x;
+        const int f1 = c1 < 0 ? 0 : A.info[c1];
+        if (c1 < 0 || info_kept(f1)) {
+            emit(c1, f1);
             continue;
         }
-        const int mid[2] = {A.left[c1], A.right[c1]};
+        const int2 mid = A.child[c1];
         for (int b = 0; b < 2; ++b) {
-            const int c2 = mid[b];
-            if (terminal(c2)) {
-                emit(c2);
+            const int c2 = b ? mid.y : mid.x;
+            const int f2 = c2 < 0 ? 0 : A.info[c2];
+            if (c2 < 0 || info_kept(f2)) {
+                emit(c2, f2);
                 continue;
             }
-            emit(A.left[c2]);
-            emit(A.right[c2]);
+            const int2 low = A.child[c2];
+            emit(low.x, low.x < 0 ? 0 : A.info[low.x]);
+            emit(low.y, low.y < 0 ? 0 : A.info[low.y]);
         }
     }
     for (int s = slot; s < 8; ++s) B.code[s] = kEmpty;
@@ -746,10 +752,10 @@ __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
 // prefix differences, then (COM, G*M) rounded once to FP32.
 __global__ __launch_bounds__(256) void cell_com_kernel(const DD* __restrict__ prefix, int n,
                                                        const int* __restrict__ first, const int* __restrict__ last,
-                                                       const int* __restrict__ kept, int m, double G,
+                                                       const int* __restrict__ info, int m, double G,
                                                        float4* __restrict__ com) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= m || !kept[i]) return;
+    if (i >= m || !info_kept(info[i])) return;
     const size_t stride = static_cast<size_t>(n) + 1;
     const int f = first[i], l = last[i] + 1;
     double r[4];
@@ -764,13 +770,13 @@ __global__ __launch_bounds__(256) void cell_com_kernel(const DD* __restrict__ pr
 }
 
 // The exported description of every radix-tree node that is an octree cell
-// (the others are left unwritten; the host keeps those with kept != 0).
+// (the others are left unwritten; the host keeps those whose word has the
+// kept bit).
 struct CellExport {
     const unsigned long long* keys;  // sorted
     const int* first;
     const int* last;
-    const int* level;
-    const int* kept;
+    const int* info;
     const float4* com;
     const DD* prefix;  // [4][n + 1]
     const RootParams* root;
@@ -780,9 +786,9 @@ struct CellExport {
 
 __global__ __launch_bounds__(256) void export_cells_kernel(CellExport E) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= E.n - 1 || !E.kept[i]) return;
+    if (i >= E.n - 1 || !info_kept(E.info[i])) return;
     const RootParams R = *E.root;
-    const int L = E.level[i], f = E.first[i], l = E.last[i];
+    const int L = info_level(E.info[i]), f = E.first[i], l = E.last[i];
     const unsigned long long cell = E.keys[f] >> (3 * (kLevels - L));  // the key's first L groups
     nbody_bh_cell c{};
     const double side = ldexp(static_cast<double>(R.size), -L);
@@ -814,7 +820,7 @@ __global__ __launch_bounds__(256) void gather_kernel(const float4* __restrict__ 
 struct TravArgs {
     const float4* node_com;
     const float* thr;  // opening threshold per level
-    const int* node_level;
+    const int* node_info;
     const unsigned* order;
     const float4* pos_in;   // by particle
     float4* pos_out;        // kick: by particle
@@ -979,7 +985,7 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         // the root: approximated by the lanes beyond its threshold (the
         // reference's side / |p - COM| < theta as r2 >= thr, see
         // open_threshold), opened by the others
-        const int root_level = A.node_level[0];
+        const int root_level = info_level(A.node_info[0]);
         const float4 c = A.node_com[0];
         const float dx = me.x - c.x, dy = me.y - c.y, dz = me.z - c.z;
         const float r2 = (dx * dx + dy * dy) + dz * dz;
@@ -1164,14 +1170,10 @@ struct BHState {
     unsigned* hist = nullptr;
     size_t hist_elems = 0;
     float4* pos_sorted = nullptr;
-    int* left = nullptr;
-    int* right = nullptr;
-    int* parent = nullptr;
-    int* leaf_parent = nullptr;
+    int2* child = nullptr;  // radix-tree nodes: encoded (left, right) children
     int* first = nullptr;
     int* last = nullptr;
-    int* level = nullptr;
-    int* kept = nullptr;
+    int* info = nullptr;    // level | kKeptBit
     ChildBlock* blocks = nullptr;
     unsigned* chunk_sums = nullptr;  // sort scan
     size_t chunk_elems = 0;
@@ -1206,14 +1208,10 @@ void bh_release(BHState* b) {
     bh_free(b->idx[1]);
     bh_free(b->hist);
     bh_free(b->pos_sorted);
-    bh_free(b->left);
-    bh_free(b->right);
-    bh_free(b->parent);
-    bh_free(b->leaf_parent);
+    bh_free(b->child);
     bh_free(b->first);
     bh_free(b->last);
-    bh_free(b->level);
-    bh_free(b->kept);
+    bh_free(b->info);
     bh_free(b->blocks);
     bh_free(b->chunk_sums);
     bh_free(b->prefix);
@@ -1333,14 +1331,10 @@ nbody_status bh_resize(BHState* b, size_t n, nbody_sim* s) {
     NB_TRY(bh_alloc(b->hist, 256 * nb));
     b->hist_elems = 256 * nb;
     NB_TRY(bh_alloc(b->pos_sorted, n));
-    NB_TRY(bh_alloc(b->left, m));
-    NB_TRY(bh_alloc(b->right, m));
-    NB_TRY(bh_alloc(b->parent, m));
-    NB_TRY(bh_alloc(b->leaf_parent, n));
+    NB_TRY(bh_alloc(b->child, m));
     NB_TRY(bh_alloc(b->first, m));
     NB_TRY(bh_alloc(b->last, m));
-    NB_TRY(bh_alloc(b->level, m));
-    NB_TRY(bh_alloc(b->kept, m));
+    NB_TRY(bh_alloc(b->info, m));
     NB_TRY(bh_alloc(b->blocks, m));
     NB_TRY(bh_alloc(b->chunk_sums, static_cast<size_t>(blocks_for(256 * nb, kScanChunk))));
     NB_TRY(bh_alloc(b->prefix, 4 * (n + 1)));
@@ -1394,17 +1388,16 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     const unsigned* order = b->idx[cur];
     hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, pos, order, n, b->pos_sorted);
     const int m = n - 1;
-    TreeArgs T{keys, n, b->left, b->right, b->parent, b->leaf_parent, b->first, b->last, b->level};
+    TreeArgs T{keys, n, b->child, b->first, b->last, b->info};
     hipLaunchKernelGGL(radix_tree_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, T);
-    hipLaunchKernelGGL(kept_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->level, b->parent, m, b->kept);
     const int mchunks = blocks_for(n + 1, kMassChunk);
     MassArgs M{b->pos_sorted, order, s->mass_dev, n, b->prefix, b->mass_chunk, mchunks};
     hipLaunchKernelGGL(mass_reduce_kernel, dim3(mchunks), dim3(kMassThreads), 0, st, M);
     hipLaunchKernelGGL(mass_chunks_kernel, dim3(1), dim3(kMassThreads), 0, st, M);
     hipLaunchKernelGGL(mass_prefix_kernel, dim3(mchunks), dim3(kMassThreads), 0, st, M);
     hipLaunchKernelGGL(cell_com_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->prefix, n, b->first, b->last,
-                       b->kept, m, s->cfg.G, b->com);
-    BlockArgs L{b->left, b->right, b->level, b->kept, b->com, b->thr, b->pos_sorted, m, b->blocks};
+                       b->info, m, s->cfg.G, b->com);
+    BlockArgs L{b->child, b->info, b->com, b->thr, b->pos_sorted, m, b->blocks};
     b->built = true;
     hipLaunchKernelGGL(block_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, L);
     NB_HIP(hipGetLastError());
@@ -1450,7 +1443,7 @@ nbody_status bh_step(nbody_sim* s, float dt, bool kick, hipEvent_t ev_b, hipEven
     TravArgs A{};
     A.node_com = b->com;
     A.thr = b->thr;
-    A.node_level = b->level;
+    A.node_info = b->info;
     A.order = b->idx[b->sorted_buf];
     A.pos_in = s->pos[s->cur];
     A.pos_out = s->pos[s->cur ^ 1];
@@ -1505,21 +1498,21 @@ nbody_status bh_export_cells(nbody_sim* s, nbody_bh_cell* out, size_t capacity, 
     const size_t m = n - 1;
     nbody_bh_cell* d = nullptr;
     NB_HIP(hipMalloc(reinterpret_cast<void**>(&d), m * sizeof(nbody_bh_cell)));
-    CellExport E{b->keys[b->sorted_buf], b->first, b->last, b->level, b->kept, b->com, b->prefix, b->root,
+    CellExport E{b->keys[b->sorted_buf], b->first, b->last, b->info, b->com, b->prefix, b->root,
                  static_cast<int>(n), d};
     hipLaunchKernelGGL(export_cells_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, s->stream, E);
     std::vector<nbody_bh_cell> cells(m);
-    std::vector<int> kept(m);
+    std::vector<int> info(m);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
         e = hipMemcpyAsync(cells.data(), d, m * sizeof(nbody_bh_cell), hipMemcpyDeviceToHost, s->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(kept.data(), b->kept, m * sizeof(int), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(info.data(), b->info, m * sizeof(int), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     (void)hipFree(d);
     NB_HIP(e);
     std::vector<nbody_bh_cell> list;
     for (size_t i = 0; i < m; ++i)
-        if (kept[i]) list.push_back(cells[i]);
+        if (info_kept(info[i])) list.push_back(cells[i]);
     // depth-first preorder of the octree: by first particle, enclosing cells first
     std::sort(list.begin(), list.end(), [](const nbody_bh_cell& x, const nbody_bh_cell& y) {
         return x.first != y.first ? x.first < y.first : x.count > y.count;
