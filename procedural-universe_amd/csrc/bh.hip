@@ -402,49 +402,105 @@ __global__ void publish_counters_kernel(const unsigned* __restrict__ counters,
     if (threadIdx.x >= 3 && threadIdx.x < 5) host[threadIdx.x] = counters[threadIdx.x - 3];
 }
 
+// One radix pass over a tile of kSortTile keys, stable.  Each wavefront owns a
+// contiguous share of the tile (kSortItems rounds of 64 consecutive keys) and
+// ranks every key among the earlier keys of its
+// wavefront with the same digit (the lanes with equal digits found by 8
+// ballots, running per-digit counts in LDS; no workgroup barrier).  One pass
+// over the 256 digits then gives each wavefront's offset within the tile's
+// digit run and the runs' starts; the keys are placed in LDS in digit order
+// and written out in that order, so consecutive threads write consecutive
+// addresses within a digit's run of the output.
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kSortPerWave = kSortTile / kSortWaves;  // = 64 * kSortItems
+
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const unsigned long long* __restrict__ keys_in, const unsigned* __restrict__ idx_in,
     unsigned long long* __restrict__ keys_out, unsigned* __restrict__ idx_out, int n, int shift,
     const unsigned* __restrict__ offsets, int nblocks) {
-    constexpr int W = kSortThreads / 64;
-    __shared__ unsigned cnt[W][256];
-    __shared__ unsigned running[256];
+    static_assert(kSortPerWave == 64 * kSortItems, "a wavefront's share is kSortItems rounds of 64 keys");
+    __shared__ unsigned cnt[kSortWaves][256];  // per-wave digit counts, then per-wave offsets in the run
+    __shared__ unsigned run_start[256];        // the tile's digit runs
+    __shared__ unsigned dst_base[256];         // the runs' places in the output
+    __shared__ unsigned wave_sum[kSortWaves];
+    __shared__ unsigned long long skeys[kSortTile];
+    __shared__ unsigned sidx[kSortTile];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    running[tid] = offsets[tid * nblocks + blockIdx.x];
     const unsigned long long below = (1ULL << lane) - 1ULL;
     const int base = blockIdx.x * kSortTile;
-    for (int k = 0; k < kSortItems; ++k) {
-        const int i = base + k * kSortThreads + tid;
+    const int wbase = base + wave * kSortPerWave;
+#pragma unroll
+    for (int d = lane; d < 256; d += 64) cnt[wave][d] = 0u;  // this wave's counters only
+    unsigned long long key[kSortItems];
+    unsigned idx[kSortItems], rank[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int i = wbase + r * 64 + lane;
         const bool valid = i < n;
-        const unsigned long long key = valid ? keys_in[i] : 0ULL;
-        const unsigned digit = static_cast<unsigned>((key >> shift) & 255u);
-        unsigned long long peers = __ballot(valid);
+        key[r] = valid ? keys_in[i] : 0ULL;
+        idx[r] = valid ? idx_in[i] : 0u;
+        const unsigned digit = static_cast<unsigned>((key[r] >> shift) & 255u);
+        unsigned long long peers = __builtin_amdgcn_ballot_w64(valid);
+#pragma unroll
         for (int bit = 0; bit < 8; ++bit) {
             const bool set = (digit >> bit) & 1u;
-            const unsigned long long b = __ballot(valid && set);
-            peers &= set ? b : ~b;
+            const unsigned long long bm = __builtin_amdgcn_ballot_w64(valid && set);
+            peers &= set ? bm : ~bm;
         }
-        const unsigned rank = static_cast<unsigned>(__popcll(peers & below));
-        for (int w = 0; w < W; ++w) cnt[w][tid] = 0;
-        __syncthreads();
-        if (valid && rank == 0) cnt[wave][digit] = static_cast<unsigned>(__popcll(peers));
-        __syncthreads();
-        {
-            unsigned run = running[tid];
-            for (int w = 0; w < W; ++w) {
-                const unsigned c = cnt[w][tid];
-                cnt[w][tid] = run;
-                run += c;
-            }
-            running[tid] = run;
+        const unsigned within = static_cast<unsigned>(__popcll(peers & below));
+        const unsigned prior = valid ? cnt[wave][digit] : 0u;  // read by every lane before the leader's update
+        rank[r] = prior + within;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && within == 0) cnt[wave][digit] = prior + static_cast<unsigned>(__popcll(peers));
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    {
+        // digit tid: offsets of the wavefronts' shares within its run, the
+        // run's length, then the runs' starts (an exclusive scan over digits)
+        unsigned run = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) {
+            const unsigned c = cnt[w][tid];
+            cnt[w][tid] = run;
+            run += c;
         }
-        __syncthreads();
-        if (valid) {
-            const unsigned dst = cnt[wave][digit] + rank;
-            keys_out[dst] = key;
-            idx_out[dst] = idx_in[i];
+        unsigned x = run;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
         }
+        if (lane == 63) wave_sum[wave] = x;
+        dst_base[tid] = offsets[tid * nblocks + blockIdx.x];
         __syncthreads();
+        unsigned start = x - run;
+        for (int w = 0; w < wave; ++w) start += wave_sum[w];
+        run_start[tid] = start;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int i = wbase + r * 64 + lane;
+        if (i < n) {
+            const unsigned digit = static_cast<unsigned>((key[r] >> shift) & 255u);
+            const unsigned at = run_start[digit] + cnt[wave][digit] + rank[r];
+            skeys[at] = key[r];
+            sidx[at] = idx[r];
+        }
+    }
+    __syncthreads();
+    const int count = min(kSortTile, n - base);
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const int at = r * kSortThreads + tid;
+        if (at < count) {
+            const unsigned long long k = skeys[at];
+            const unsigned digit = static_cast<unsigned>((k >> shift) & 255u);
+            const unsigned dst = dst_base[digit] + (static_cast<unsigned>(at) - run_start[digit]);
+            keys_out[dst] = k;
+            idx_out[dst] = sidx[at];
+        }
     }
 }
 
