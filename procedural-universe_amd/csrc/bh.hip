@@ -16,8 +16,9 @@
 //                    for bit and the cells are the reference's half-open cubes;
 //  3. sort           least-significant-digit radix sort of (key, index), eight
 //                    8-bit passes of histogram / scan / scatter; the scatter
-//                    ranks equal digits with wave ballots in LDS, so the order
-//                    is stable and equals std::stable_sort of the keys;
+//                    ranks equal digits with wave ballots, orders each tile by
+//                    digit in LDS and writes it out in runs, so the order is
+//                    stable and equals std::stable_sort of the keys;
 //  4. radix tree     binary radix tree over the sorted keys (Karras 2012);
 //                    a node whose common prefix reaches a deeper 3-bit group
 //                    than its parent's is an octree cell ("kept"), the others
@@ -25,7 +26,10 @@
 //                    form a level-21 bucket;
 //  5. mass           FP64 mass and mass-weighted position of every node as a
 //                    difference of double-double prefix sums over the sorted
-//                    particles (a node is a contiguous range of them);
+//                    particles (a node is a contiguous range of them); then
+//                    every cell writes one 192-byte record of its children
+//                    (at most eight, in key order): sources, opening
+//                    thresholds and codes, all the traversal reads of it;
 //  6. traversal      one lane per particle in Morton order; a wave walks the
 //                    octree with one stack in LDS of (cell, lanes) entries.
 //                    Each lane makes the reference's own decision at each cell
