@@ -158,6 +158,50 @@ def test_identical_positions_form_a_bucket(gpu):
     assert rel_err_rows(a_bh, a_d).max() < 1e-5  # two FP32 sums of the same pairs in different orders
 
 
+def test_bucket_acts_as_one_particle_of_its_total_mass(gpu):
+    """At theta 0.5 a bucket of 61 coincident particles has the centre of mass,
+    mass, cells and opening decisions of one particle of their total mass at
+    that point: every other particle feels the same force, and the coincident
+    ones (same position, same decisions, zero contribution from each other)
+    get the single particle's acceleration (equal to rounding: lanes in two
+    wavefronts flush their lists at different points)."""
+    pu = gpu
+    n, soft = 1000, 1e-4
+    pos, vel, mass = pu.ic_plummer(n, 0x5EED000B)
+    pos[200:260] = pos[3]
+    cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=soft, theta=0.5)
+    with _bh(pu, cfg, pos, vel, mass) as sim:
+        a_bucket = sim.accelerations()
+    keep = np.r_[0:200, 260:n]
+    mass_one = mass[keep].copy()
+    mass_one[3] = mass[3] + mass[200:260].sum()
+    with _bh(pu, cfg, pos[keep], vel[keep], mass_one) as sim:
+        a_one = sim.accelerations()
+    coincident = a_bucket[np.r_[3, 200:260]]
+    assert np.all(np.isfinite(a_bucket))
+    assert rel_err_rows(coincident, np.broadcast_to(coincident[0], coincident.shape)).max() < 1e-6
+    err = rel_err_rows(a_bucket[keep], a_one)
+    print(f"bucket vs one particle: max rel {err.max():.2e}")
+    assert err.max() < 1e-5
+
+
+def test_all_particles_at_one_point(gpu):
+    """Every key equal: the root itself is a bucket (or, with a root side near
+    2^-99, a monopole at the particles' own position), every pair has d = 0,
+    and every acceleration is exactly zero."""
+    pu = gpu
+    n = 50
+    pos = np.tile(np.array([[0.25, -0.5, 1.0]], np.float32), (n, 1))
+    vel = np.zeros((n, 3), np.float32)
+    mass = np.full(n, 1.0 / n)
+    cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=1e-4, theta=0.5)
+    with _bh(pu, cfg, pos, vel, mass) as sim:
+        a = sim.accelerations()
+        st = sim.bh_stats()
+    assert np.all(a == 0.0)
+    assert st["stack_overflow"] == 0
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 17])
 def test_tiny_sizes(gpu, oracle, n):
     pu = gpu
