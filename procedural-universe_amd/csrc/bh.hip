@@ -358,35 +358,6 @@ __global__ __launch_bounds__(kScanThreads2) void scan_apply_kernel(unsigned* __r
     }
 }
 
-// The whole digit-count table in one workgroup when it is small (up to 64
-// counts per thread, each thread a contiguous run).
-constexpr int kScanSmallThreads = 1024;
-constexpr int kScanSmallMax = kScanSmallThreads * 64;
-
-__global__ __launch_bounds__(kScanSmallThreads) void scan_small_kernel(unsigned* __restrict__ data, int total) {
-    __shared__ unsigned wsum[kScanSmallThreads / 64];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int per = (total + kScanSmallThreads - 1) / kScanSmallThreads;
-    const int b = min(t * per, total), e = min(b + per, total);
-    unsigned s = 0;
-    for (int i = b; i < e; ++i) s += data[i];
-    unsigned x = s;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const unsigned y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    if (lane == 63) wsum[wave] = x;
-    __syncthreads();
-    unsigned run = x - s;
-    for (int w = 0; w < wave; ++w) run += wsum[w];
-    for (int i = b; i < e; ++i) {
-        const unsigned v = data[i];
-        data[i] = run;
-        run += v;
-    }
-}
-
 // Counters: walk[3] (interactions, list entries, cells opened) and
 // counters[2] (outside, overflow) on the device, mirrored after every force
 // pass into pinned host memory (host[5], same order), so reading them needs
@@ -408,25 +379,37 @@ __global__ void publish_counters_kernel(const unsigned* __restrict__ counters,
 
 // One radix pass over a tile of kSortTile keys, stable.  Each wavefront owns a
 // contiguous share of the tile (kSortItems rounds of 64 consecutive keys) and
-// ranks every key among the earlier keys of its
-// wavefront with the same digit (the lanes with equal digits found by 8
-// ballots, running per-digit counts in LDS; no workgroup barrier).  One pass
-// over the 256 digits then gives each wavefront's offset within the tile's
-// digit run and the runs' starts; the keys are placed in LDS in digit order
-// and written out in that order, so consecutive threads write consecutive
-// addresses within a digit's run of the output.
+// ranks every key among the earlier keys of its wavefront with the same digit
+// (the lanes with equal digits found by 8 ballots, running per-digit counts in
+// LDS; no workgroup barrier).  One pass over the 256 digits then gives each
+// wavefront's offset within the tile's digit run and the runs' starts; the
+// keys are placed in LDS in digit order and written out in that order, so
+// consecutive threads write consecutive addresses within a digit's run of the
+// output.
+//
+// FUSED: `counts` holds the raw per-tile digit counts ([digit * nblocks +
+// tile], at most kFusedScanTiles tiles) and each workgroup forms its own
+// output places: thread d sums digit d's row before its tile and in total,
+// and one scan over the digits' totals gives the digits' starts (no separate
+// scan launch).  Otherwise `counts` already holds the exclusive prefix of the
+// counts.  GATHER (the last pass): the sorted positions are written beside the
+// keys.
 constexpr int kSortWaves = kSortThreads / 64;
 constexpr int kSortPerWave = kSortTile / kSortWaves;  // = 64 * kSortItems
 
+constexpr int kFusedScanTiles = 256;
+
+template <bool FUSED, bool GATHER>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const unsigned long long* __restrict__ keys_in, const unsigned* __restrict__ idx_in,
     unsigned long long* __restrict__ keys_out, unsigned* __restrict__ idx_out, int n, int shift,
-    const unsigned* __restrict__ offsets, int nblocks) {
+    const unsigned* __restrict__ counts, int nblocks, const float4* __restrict__ pos_in,
+    float4* __restrict__ pos_out) {
     static_assert(kSortPerWave == 64 * kSortItems, "a wavefront's share is kSortItems rounds of 64 keys");
     __shared__ unsigned cnt[kSortWaves][256];  // per-wave digit counts, then per-wave offsets in the run
     __shared__ unsigned run_start[256];        // the tile's digit runs
     __shared__ unsigned dst_base[256];         // the runs' places in the output
-    __shared__ unsigned wave_sum[kSortWaves];
+    __shared__ unsigned wave_sum[2][kSortWaves];
     __shared__ unsigned long long skeys[kSortTile];
     __shared__ unsigned sidx[kSortTile];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -469,18 +452,41 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
             cnt[w][tid] = run;
             run += c;
         }
-        unsigned x = run;
+        // digit tid's place in the output: the exclusive prefix of the
+        // counts (given, or formed here from the raw counts)
+        unsigned before = 0, total = 0;
+        if constexpr (FUSED) {
+            const unsigned* row = counts + static_cast<size_t>(tid) * nblocks;
+            for (int b = 0; b < nblocks; ++b) {
+                const unsigned c = row[b];
+                before += b < static_cast<int>(blockIdx.x) ? c : 0u;
+                total += c;
+            }
+        } else {
+            before = counts[tid * nblocks + blockIdx.x];
+        }
+        unsigned x = run, y = total;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
-            const unsigned y = __shfl_up(x, off, 64);
-            if (lane >= off) x += y;
+            const unsigned xs = __shfl_up(x, off, 64);
+            const unsigned ys = __shfl_up(y, off, 64);
+            if (lane >= off) {
+                x += xs;
+                y += ys;
+            }
         }
-        if (lane == 63) wave_sum[wave] = x;
-        dst_base[tid] = offsets[tid * nblocks + blockIdx.x];
+        if (lane == 63) {
+            wave_sum[0][wave] = x;
+            wave_sum[1][wave] = y;
+        }
         __syncthreads();
-        unsigned start = x - run;
-        for (int w = 0; w < wave; ++w) start += wave_sum[w];
+        unsigned start = x - run, digit_start = y - total;
+        for (int w = 0; w < wave; ++w) {
+            start += wave_sum[0][w];
+            digit_start += wave_sum[1][w];
+        }
         run_start[tid] = start;
+        dst_base[tid] = FUSED ? digit_start + before : before;
     }
     __syncthreads();
 #pragma unroll
@@ -504,6 +510,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
             const unsigned dst = dst_base[digit] + (static_cast<unsigned>(at) - run_start[digit]);
             keys_out[dst] = k;
             idx_out[dst] = sidx[at];
+            if constexpr (GATHER) pos_out[dst] = pos_in[sidx[at]];
         }
     }
 }
@@ -867,12 +874,6 @@ __global__ __launch_bounds__(256) void export_cells_kernel(CellExport E) {
     c.first = static_cast<unsigned>(f);
     c.count = static_cast<unsigned>(l - f + 1);
     E.out[i] = c;
-}
-
-__global__ __launch_bounds__(256) void gather_kernel(const float4* __restrict__ pos, const unsigned* __restrict__ order,
-                                                     int n, float4* __restrict__ out) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k < n) out[k] = pos[order[k]];
 }
 
 // ---- 6. traversal ---------------------------------------------------------------------
@@ -1430,23 +1431,24 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
         const int shift = 8 * pass;
         hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], n, shift, b->hist, nb);
         const int total = 256 * nb, nchunks = blocks_for(total, kScanChunk);
-        if (total <= kScanSmallMax) {
-            hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(kScanSmallThreads), 0, st, b->hist, total);
-        } else {
+        const bool fused = nb <= kFusedScanTiles;
+        if (!fused) {
             hipLaunchKernelGGL(scan_reduce_kernel, dim3(nchunks), dim3(kScanThreads2), 0, st, b->hist, total,
                                b->chunk_sums);
             hipLaunchKernelGGL(scan_chunks_kernel, dim3(1), dim3(kScanThreads2), 0, st, b->chunk_sums, nchunks);
             hipLaunchKernelGGL(scan_apply_kernel, dim3(nchunks), dim3(kScanThreads2), 0, st, b->hist, total,
                                b->chunk_sums);
         }
-        hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], b->idx[cur],
-                           b->keys[cur ^ 1], b->idx[cur ^ 1], n, shift, b->hist, nb);
+        // the last pass also writes the sorted positions
+        auto scatter = pass < 7 ? (fused ? radix_scatter_kernel<true, false> : radix_scatter_kernel<false, false>)
+                                : (fused ? radix_scatter_kernel<true, true> : radix_scatter_kernel<false, true>);
+        hipLaunchKernelGGL(scatter, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], b->idx[cur], b->keys[cur ^ 1],
+                           b->idx[cur ^ 1], n, shift, b->hist, nb, pos, b->pos_sorted);
         cur ^= 1;
     }
     b->sorted_buf = cur;
     const unsigned long long* keys = b->keys[cur];
     const unsigned* order = b->idx[cur];
-    hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, pos, order, n, b->pos_sorted);
     const int m = n - 1;
     TreeArgs T{keys, n, b->child, b->first, b->last, b->info};
     hipLaunchKernelGGL(radix_tree_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, T);
