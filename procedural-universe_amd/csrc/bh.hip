@@ -70,7 +70,7 @@ constexpr int kBucketLevel = kLevels;
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
-constexpr int kTravThreads = 256;
+constexpr int kTravThreads = 64;  // one wavefront per workgroup: the finest tail
 constexpr int kTravWaves = kTravThreads / 64;
 constexpr int kStackCap = 176;  // > 7 * 22 + 8 pending cells per wave
 constexpr int kEmpty = -1;
@@ -971,7 +971,7 @@ __device__ __forceinline__ void flush_list(const ListEntry* lst, int cnt, int la
 constexpr int kXcds = 8;
 
 __device__ __forceinline__ int xcd_block(int b, int nb) {
-    constexpr int C = 8;
+    constexpr int C = 2048 / kTravThreads;  // runs of 2048 particles
     const int full = (nb / (kXcds * C)) * (kXcds * C);
     if (b >= full) return b;
     const int x = b % kXcds, k = b / kXcds;
