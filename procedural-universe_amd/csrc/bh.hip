@@ -544,8 +544,8 @@ struct TreeArgs {
     int* info;    // [n-1] level | kKeptBit
 };
 
-__global__ __launch_bounds__(256) void radix_tree_kernel(TreeArgs T) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+__device__ void radix_tree_block(const TreeArgs& T, int block) {
+    const int i = block * 256 + threadIdx.x;
     const int n = T.n;
     if (i >= n - 1) return;
     const unsigned long long* keys = T.keys;
@@ -735,12 +735,12 @@ constexpr int kMassWaves = kMassThreads / 64;
 // scanned by wave scans plus the totals of the waves before it.
 
 // Sum of each chunk, per quantity (rows in order, a row's waves in order).
-__global__ __launch_bounds__(kMassThreads) void mass_reduce_kernel(MassArgs A) {
+__device__ void mass_reduce_block(const MassArgs& A, int block) {
     __shared__ DD wsum[4][kMassWaves];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     DD acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     for (int r = 0; r < kMassPer; ++r) {
-        const int i = blockIdx.x * kMassChunk + r * kMassThreads + threadIdx.x;
+        const int i = block * kMassChunk + r * kMassThreads + threadIdx.x;
         double v[4];
         leaf_vals(A, i, v);
 #pragma unroll
@@ -756,8 +756,17 @@ __global__ __launch_bounds__(kMassThreads) void mass_reduce_kernel(MassArgs A) {
     if (threadIdx.x < 4) {
         DD t{0.0, 0.0};
         for (int w = 0; w < kMassWaves; ++w) t = dd_add(t, wsum[threadIdx.x][w]);
-        A.chunk[threadIdx.x * A.nchunks + blockIdx.x] = t;
+        A.chunk[threadIdx.x * A.nchunks + block] = t;
     }
+}
+
+// The radix tree's nodes and the chunk sums of the mass prefix are
+// independent: one launch, the first `tree_blocks` workgroups build nodes and
+// the others reduce chunks (both read only the sorted particles).
+static_assert(kMassThreads == 256, "one workgroup size for both halves");
+__global__ __launch_bounds__(256) void tree_mass_kernel(TreeArgs T, MassArgs A, int tree_blocks) {
+    if (static_cast<int>(blockIdx.x) < tree_blocks) radix_tree_block(T, blockIdx.x);
+    else mass_reduce_block(A, blockIdx.x - tree_blocks);
 }
 
 // Exclusive scan of the chunk sums (one workgroup, quantities in turn).
@@ -784,14 +793,37 @@ __global__ __launch_bounds__(kMassThreads) void mass_chunks_kernel(MassArgs A) {
     }
 }
 
-// Exclusive prefix P[k] of every element k in [0, n]; coalesced rows.
+// Exclusive prefix P[k] of every element k in [0, n]; coalesced rows.  With
+// FUSED (at most kMassFusedChunks chunks) `chunk` holds the chunks' sums and
+// the first wavefront adds those of the earlier chunks itself (lane c sums
+// chunks c, c + 64, ... in order, then one wave scan), in place of
+// mass_chunks_kernel's exclusive scan.
+constexpr int kMassFusedChunks = 256;
+
+template <bool FUSED>
 __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
     __shared__ DD wsum[4][kMassWaves];
+    __shared__ DD carry_sh[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t stride = static_cast<size_t>(A.n) + 1;
     DD carry[4];
+    if constexpr (FUSED) {
+        if (wave == 0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) carry[q] = A.chunk[q * A.nchunks + blockIdx.x];
+            for (int q = 0; q < 4; ++q) {
+                DD x{0.0, 0.0};
+                for (int c = lane; c < static_cast<int>(blockIdx.x); c += 64) x = dd_add(x, A.chunk[q * A.nchunks + c]);
+                const DD incl = wave_scan_dd(x, lane);
+                if (lane == 63) carry_sh[q] = incl;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) carry[q] = carry_sh[q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) carry[q] = A.chunk[q * A.nchunks + blockIdx.x];
+    }
     for (int r = 0; r < kMassPer; ++r) {
         const int i = blockIdx.x * kMassChunk + r * kMassThreads + threadIdx.x;
         double v[4];
@@ -1451,12 +1483,16 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     const unsigned* order = b->idx[cur];
     const int m = n - 1;
     TreeArgs T{keys, n, b->child, b->first, b->last, b->info};
-    hipLaunchKernelGGL(radix_tree_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, T);
+    const int tree_blocks = blocks_for(m, 256);
     const int mchunks = blocks_for(n + 1, kMassChunk);
     MassArgs M{b->pos_sorted, order, s->mass_dev, n, b->prefix, b->mass_chunk, mchunks};
-    hipLaunchKernelGGL(mass_reduce_kernel, dim3(mchunks), dim3(kMassThreads), 0, st, M);
-    hipLaunchKernelGGL(mass_chunks_kernel, dim3(1), dim3(kMassThreads), 0, st, M);
-    hipLaunchKernelGGL(mass_prefix_kernel, dim3(mchunks), dim3(kMassThreads), 0, st, M);
+    hipLaunchKernelGGL(tree_mass_kernel, dim3(tree_blocks + mchunks), dim3(256), 0, st, T, M, tree_blocks);
+    if (mchunks <= kMassFusedChunks) {
+        hipLaunchKernelGGL(mass_prefix_kernel<true>, dim3(mchunks), dim3(kMassThreads), 0, st, M);
+    } else {
+        hipLaunchKernelGGL(mass_chunks_kernel, dim3(1), dim3(kMassThreads), 0, st, M);
+        hipLaunchKernelGGL(mass_prefix_kernel<false>, dim3(mchunks), dim3(kMassThreads), 0, st, M);
+    }
     hipLaunchKernelGGL(cell_com_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->prefix, n, b->first, b->last,
                        b->info, m, s->cfg.G, b->com);
     BlockArgs L{b->child, b->info, b->com, b->thr, b->pos_sorted, m, b->blocks};
