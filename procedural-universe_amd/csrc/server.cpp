@@ -46,9 +46,9 @@ namespace {
 
 // Device bytes per particle: a simulation handle (positions x2, velocity,
 // acceleration, FP64 mass and G m: 80 B) plus the Barnes-Hut tree (keys and
-// indices x2: 24 B, sorted positions: 16 B, radix-tree node arrays: 32 B,
+// indices x2: 24 B, sorted positions: 16 B, radix-tree node arrays: 20 B,
 // child blocks: 192 B, double-double prefix sums: 64 B, cell records: 16 B,
-// digit counts: under 1 B): 425 B, counted as 432.
+// digit counts: under 1 B): 413 B, counted as 432 (an upper bound).
 constexpr size_t kBhBytesPerParticle = 80 + 352;
 constexpr size_t kDirectBytesPerParticle = 64;
 
