@@ -316,6 +316,37 @@ def test_cell_export_describes_the_octree(gpu):
     assert np.all(np.diff(count)[same] < 0)
 
 
+@pytest.mark.parametrize("n", [524288, 524289, 1048575, 1048576])
+def test_sort_and_mass_paths_at_their_thresholds(gpu, oracle, n):
+    """The sort's scatter scans its own digit counts up to 256 tiles of 2048
+    keys (N <= 524,288) and uses separate scan passes above; the mass prefix
+    folds its chunk scan up to 256 chunks of 4096 (N + 1 <= 1,048,576).  On
+    both sides of both thresholds the keys and the stable order are bit-exact
+    and the cells' FP64 masses are the sums of their particles' masses."""
+    pu = gpu
+    pos, vel, mass = pu.ic_plummer(n, 0x5EED0300 + (n & 0xff))
+    cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=1e-4, theta=0.5)
+    with _bh(pu, cfg, pos, vel, mass) as sim:
+        a = sim.accelerations()
+        keys, order = sim.bh_tree_order()
+        cells = sim.bh_cells()
+        st = sim.bh_stats()
+    lo, size = oracle.dynamic_root(pos)
+    ref_keys, outside = oracle.morton_keys(pos, lo, size)
+    ref_order = oracle.sort_order(ref_keys)
+    assert outside == 0 and st["stack_overflow"] == 0
+    assert np.array_equal(order, ref_order)
+    assert np.array_equal(keys, ref_keys[ref_order])
+    assert np.all(np.isfinite(a))
+    m = mass[order]
+    csum = np.concatenate([[0.0], np.cumsum(m)])
+    root = cells[0]
+    assert root["count"] == n and root["mass"] == pytest.approx(mass.sum(), rel=1e-12)
+    for c in cells[:: max(1, cells.size // 500)]:
+        f, k = int(c["first"]), int(c["count"])
+        assert c["mass"] == pytest.approx(m[f:f + k].sum(), rel=1e-12, abs=1e-15 * csum[-1])
+
+
 def test_full_size_encounter_keys_and_tree(gpu, oracle):
     """N = 4,194,304 (two 2,097,152-particle discs on a parabolic encounter,
     theta 0.5): keys and stable order bit-exact against the CPU on the full N,
