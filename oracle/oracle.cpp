@@ -269,7 +269,9 @@ struct Tree {
     }
 
     // Octree::CalculateForce (Octree.cpp:107-145).
-    void calculate_force(int id, int32_t p, double G, double S, double theta, double* f) const {
+    // *count gains one per interaction with a node that holds mass (the empty
+    // cubes the reference also evaluates add zero force and are not counted).
+    void calculate_force(int id, int32_t p, double G, double S, double theta, double* f, int64_t* count) const {
         const Node& nd = nodes[id];
         const V3 pp = load(pos, static_cast<size_t>(p));
         if (nd.num == 1) {
@@ -280,6 +282,7 @@ struct Tree {
                 f[0] += g * static_cast<double>(diff.x);
                 f[1] += g * static_cast<double>(diff.y);
                 f[2] += g * static_cast<double>(diff.z);
+                ++*count;
             }
         } else {
             const float r = length(sub(pp, nd.com));
@@ -290,8 +293,9 @@ struct Tree {
                 f[0] += g * static_cast<double>(diff.x);
                 f[1] += g * static_cast<double>(diff.y);
                 f[2] += g * static_cast<double>(diff.z);
+                if (nd.num > 0) ++*count;
             } else if (!nd.leaf) {
-                for (int k = 0; k < 8; ++k) calculate_force(nd.child[k], p, G, S, theta, f);
+                for (int k = 0; k < 8; ++k) calculate_force(nd.child[k], p, G, S, theta, f, count);
             }
         }
     }
@@ -468,9 +472,10 @@ void oracle_bh_forces_ex(size_t n, const float* pos, const double* mass, double 
                          const float* root_lo, float root_size, int com_fp64, int workers, double* out_forces,
                          int64_t* stats) {
     auto t = build_tree(n, pos, mass, root_lo, root_size, com_fp64 != 0);
+    std::vector<int64_t> counts(n, 0);
     parallel_for(n, workers, [&](size_t i) {
         double f[3] = {0, 0, 0};
-        t->calculate_force(0, static_cast<int32_t>(i), G, S, theta, f);
+        t->calculate_force(0, static_cast<int32_t>(i), G, S, theta, f, &counts[i]);
         out_forces[3 * i] = f[0];
         out_forces[3 * i + 1] = f[1];
         out_forces[3 * i + 2] = f[2];
@@ -482,6 +487,9 @@ void oracle_bh_forces_ex(size_t n, const float* pos, const double* mass, double 
         stats[1] = leaves;
         stats[2] = t->max_depth;
         stats[3] = t->dropped;
+        int64_t total = 0;
+        for (int64_t c : counts) total += c;
+        stats[4] = total;
     }
 }
 

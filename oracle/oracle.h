@@ -51,9 +51,12 @@ void oracle_fp64_energy(int law, size_t n, const float* pos, const float* vel, c
  * inserting particles in index order (Octree::Add/Split, Octree.cpp:16-84),
  * compute node masses (Octree::CalculateMass, :86-105) and every particle's
  * force (Octree::CalculateForce, :107-145) with opening angle theta.
- * out_forces: 3n doubles (F, not a).  stats (may be NULL): [0] nodes,
- * [1] leaves holding one particle, [2] maximum depth, [3] particles dropped
- * (outside every child cube). */
+ * out_forces: 3n doubles (F, not a).  stats (may be NULL, else 5 entries):
+ * [0] nodes, [1] leaves holding one particle, [2] maximum depth, [3] particles
+ * dropped (outside every child cube), [4] interactions summed over the targets
+ * with nodes that hold mass (a single-particle leaf other than the target's
+ * own, or an accepted cell; the empty cubes the reference also evaluates add
+ * zero force and are not counted). */
 void oracle_bh_forces(size_t n, const float* pos, const double* mass, double G, double S, double theta,
                       const float* root_lo, float root_size, int workers, double* out_forces, int64_t* stats);
 

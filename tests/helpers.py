@@ -134,13 +134,14 @@ class Oracle:
         pos, mass = _f32(pos), _f64(mass)
         lo = _f32(root_lo)
         out = np.empty((mass.size, 3))
-        stats = np.zeros(4, np.int64)
+        stats = np.zeros(5, np.int64)
         self.L.oracle_bh_forces_ex(
             mass.size, pos.ctypes.data_as(_F32P), mass.ctypes.data_as(_F64P), G, S, theta,
             lo.ctypes.data_as(_F32P), float(root_size), 1 if com_fp64 else 0, workers, out.ctypes.data_as(_F64P),
             stats.ctypes.data_as(_I64P)
         )
-        return out, {"nodes": int(stats[0]), "leaves": int(stats[1]), "max_depth": int(stats[2]), "dropped": int(stats[3])}
+        return out, {"nodes": int(stats[0]), "leaves": int(stats[1]), "max_depth": int(stats[2]), "dropped": int(stats[3]),
+                     "interactions": int(stats[4])}
 
     def bh_step(self, pos, vel, mass, dt, nsteps, theta, root_lo, root_size, G=REF_G, S=REF_S, scale=REF_SCALE,
                 workers=0):

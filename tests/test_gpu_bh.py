@@ -74,11 +74,18 @@ def test_reference_octree_forces(gpu, oracle, theta):
     F, info = oracle.bh_forces(pos, mass, theta, REF_LO, REF_SIZE)
     ref = F / mass[:, None]
     err = rel_err_rows(a, ref)
+    dcount = st["interactions"] - info["interactions"]
     print(f"theta {theta}: max rel {err.max():.2e}, p99.9 {np.quantile(err, 0.999):.2e}, "
-          f"interactions/particle {st['interactions'] / n:.0f}, stack overflow {st['stack_overflow']}")
+          f"interactions/particle {st['interactions'] / n:.0f} (device - oracle {dcount}), "
+          f"stack overflow {st['stack_overflow']}")
     assert st["stack_overflow"] == 0
     assert info["dropped"] == 0
     assert err.max() < 1e-5
+    # The same walk: one interaction per accepted cell and per other particle
+    # reached.  A dropped or repeated source of small mass would pass the force
+    # bound; the count does not.  Decisions may flip only where |p - COM| is
+    # within FP32 rounding of the threshold (FP64 vs FP32 cell centres).
+    assert abs(dcount) <= 1e-5 * info["interactions"]
 
 
 def test_dynamic_root_forces(gpu, oracle):
@@ -90,11 +97,14 @@ def test_dynamic_root_forces(gpu, oracle):
     cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=soft, theta=theta)
     with _bh(pu, cfg, pos, vel, mass) as sim:
         a = sim.accelerations()
+        st = sim.bh_stats()
     lo, size = oracle.dynamic_root(pos)
-    F, _ = oracle.bh_forces(pos, mass, theta, lo, size, G=1.0, S=soft)
+    F, info = oracle.bh_forces(pos, mass, theta, lo, size, G=1.0, S=soft)
     err = rel_err_rows(a, F / mass[:, None])
-    print(f"dynamic root: max rel {err.max():.2e}")
+    dcount = st["interactions"] - info["interactions"]
+    print(f"dynamic root: max rel {err.max():.2e}, interactions device - oracle {dcount}")
     assert err.max() < 1e-5
+    assert abs(dcount) <= 1e-5 * info["interactions"]
 
 
 def test_theta_zero_is_direct_sum(gpu):
@@ -247,20 +257,23 @@ def test_two_disc_accuracy_vs_direct(gpu, oracle):
         a_d = sim.accelerations()
     lo, size = oracle.dynamic_root(pos)
     # same cells and opening test with FP64 centres of mass: the traversal itself
-    F64, _ = oracle.bh_forces(pos, mass, 0.5, lo, size, G=1.0, S=soft, com_fp64=True)
+    F64, info64 = oracle.bh_forces(pos, mass, 0.5, lo, size, G=1.0, S=soft, com_fp64=True)
     perr = rel_err_rows(a_bh, F64 / mass[:, None])
     # the reference's FP32 centre-of-mass accumulation differs by its own rounding
-    F32, _ = oracle.bh_forces(pos, mass, 0.5, lo, size, G=1.0, S=soft)
+    F32, info32 = oracle.bh_forces(pos, mass, 0.5, lo, size, G=1.0, S=soft)
     rerr = rel_err_rows(a_bh, F32 / mass[:, None])
     err = rel_err_rows(a_bh, a_d)
     rms = np.sqrt(np.mean(err ** 2))
     print(f"two discs N={n}: vs CPU octree (FP64 COM) max {perr.max():.2e}; vs reference FP32 COM "
           f"rms {np.sqrt(np.mean(rerr ** 2)):.2e} max {rerr.max():.2e}; vs direct rms {rms:.2e} max {err.max():.2e}; "
-          f"interactions/particle {st['interactions'] / n:.0f}")
+          f"interactions/particle {st['interactions'] / n:.0f} (device - oracle {st['interactions'] - info64['interactions']} "
+          f"with FP64 COM, {st['interactions'] - info32['interactions']} with FP32 COM)")
     assert perr.max() < 1e-5
     assert np.sqrt(np.mean(rerr ** 2)) < 1e-4
     assert rms < 3e-2
     assert st["interactions"] < n * n / 4
+    # the same walk as the octree with the same centres of mass
+    assert abs(st["interactions"] - info64["interactions"]) <= 1e-5 * info64["interactions"]
 
 
 @pytest.mark.parametrize("k", [-2, -1, 0, 1, 2, 3])
@@ -273,9 +286,11 @@ def test_opening_decision_at_the_boundary(gpu, oracle, k):
     cfg = pu.reference_config(pu.METHOD_BARNES_HUT, theta=0.5)
     with _bh(pu, cfg, pos, vel, mass) as sim:
         a = sim.accelerations()
-    F, _ = oracle.bh_forces(pos, mass, 0.5, REF_LO, REF_SIZE)
+        st = sim.bh_stats()
+    F, info = oracle.bh_forces(pos, mass, 0.5, REF_LO, REF_SIZE)
     err = rel_err_rows(a, F / mass[:, None])
     assert err.max() < 1e-5, err
+    assert st["interactions"] == info["interactions"] == (5 if k >= 1 else 6)
 
 
 def test_cell_export_describes_the_octree(gpu):

@@ -293,6 +293,7 @@ def test_bh_theta_zero_is_direct_sum(oracle, pu):
     F_bh, st = oracle.bh_forces(pos, mass, 0.0, [-4000] * 3, 8000.0)
     F = oracle.ref_forces(pos, mass)
     assert st["dropped"] == 0 and st["leaves"] == 300
+    assert st["interactions"] == 300 * 299
     assert np.max(rel_err_rows(F_bh, F)) < 1e-12
 
 
@@ -300,7 +301,8 @@ def test_bh_large_theta_is_root_monopole(oracle, pu):
     """A huge theta approximates at the root: F_p = Gravity(p, COM, M) along (COM - p),
     the root monopole including p itself (no self-exclusion, Octree.cpp:122-132)."""
     pos, vel, mass = pu.ic_uniform(50, 23, half=200.0)
-    F_bh, _ = oracle.bh_forces(pos, mass, 1e30, [-4000] * 3, 8000.0)
+    F_bh, st = oracle.bh_forces(pos, mass, 1e30, [-4000] * 3, 8000.0)
+    assert st["interactions"] == 50  # one root monopole per target
     M = mass.sum()
     com = (pos.astype(np.float64) * mass[:, None]).sum(0) / M
     for i in range(50):
@@ -337,6 +339,9 @@ def test_bh_opening_test_at_the_boundary(oracle, k):
     pos, mass, t = opening_boundary_case(k)
     F, st = oracle.bh_forces(pos, mass, 0.5, [-4000] * 3, 8000.0)
     assert st["dropped"] == 0
+    # particles 0 and 1 each meet the other and the target's leaf; the target
+    # meets the pair's two leaves, or their cell's monopole once
+    assert st["interactions"] == (5 if k >= 1 else 6)
     exact = law_r_force(pos[t], pos[0], mass[t], mass[0]) + law_r_force(pos[t], pos[1], mass[t], mass[1])
     mono = law_r_force(pos[t], [62.5, 62.5, 62.5], mass[t], mass[0] + mass[1])
     assert np.linalg.norm(exact - mono) > 1e-2 * np.linalg.norm(exact)
