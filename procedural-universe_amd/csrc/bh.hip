@@ -72,7 +72,10 @@ constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kTravThreads = 64;  // one wavefront per workgroup: the finest tail
 constexpr int kTravWaves = kTravThreads / 64;
-constexpr int kStackCap = 176;  // > 7 * 22 + 8 pending cells per wave
+// Pending (cell, lanes) entries per wave.  A batch of k cells replaces them by
+// at most 8k children, so a batch is only as large as the room left allows;
+// a single cell beyond that goes through a checked push that reports overflow.
+constexpr int kStackCap = 224;
 constexpr int kEmpty = -1;
 
 // Child encoding: >= 0 internal node; <= -2 particle at sorted position (-c - 2).
@@ -958,15 +961,18 @@ __device__ __forceinline__ float lane_gm(float gm, unsigned w, int lane) {
     return __int_as_float(__float_as_int(gm) & __builtin_amdgcn_sbfe(static_cast<int>(w), lane & 31, 1));
 }
 
-// (lo, hi) = x in lane L only (register writes; the other lanes keep lo and
-// hi).  x is usually a compare result: a VALU write of VCC read by a VALU
-// instruction as a scalar operand needs one wait state, of another SGPR pair
-// two, and the compiler does not look inside the asm, so the asm waits two.
-template <int L>
-__device__ __forceinline__ void write_lane64(unsigned& lo, unsigned& hi, unsigned long long x) {
-    asm("s_nop 1\n\tv_writelane_b32 %0, %2, %4\n\tv_writelane_b32 %1, %3, %4"
+// (lo, hi) = x in lane l only (register writes; the other lanes keep lo and
+// hi).  The lane number goes through M0: one instruction may read only one
+// SGPR besides M0.  The compiler does not look inside the asm, so the asm
+// waits for the hazards its operands can carry: x is usually a compare result
+// (a VALU write of VCC read by a VALU instruction as a scalar operand needs
+// one wait state, of another SGPR pair two), and a lane select written just
+// before needs up to four.
+__device__ __forceinline__ void write_lane64(unsigned& lo, unsigned& hi, unsigned long long x, int l) {
+    asm("s_mov_b32 m0, %4\n\ts_nop 4\n\tv_writelane_b32 %0, %2, m0\n\tv_writelane_b32 %1, %3, m0"
         : "+v"(lo), "+v"(hi)
-        : "s"(static_cast<unsigned>(x)), "s"(static_cast<unsigned>(x >> 32)), "n"(L));
+        : "s"(static_cast<unsigned>(x)), "s"(static_cast<unsigned>(x >> 32)), "s"(l)
+        : "m0");
 }
 
 template <int LAW>
@@ -1093,18 +1099,29 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
             sp = 1;
         }
     }
+    // Lane 8j + s holds child s of the batch's j-th cell (j = 0 the top of
+    // the stack) in the write phase.
+    const int grp = lane >> 3;
     while (sp > 0) {
-        --sp;
         __builtin_amdgcn_wave_barrier();
-        const int code = __builtin_amdgcn_readfirstlane(stk_node[wave][sp]);
-        const uint2 mv = stk_mask[wave][sp];
-        const unsigned lo32 = __builtin_amdgcn_readfirstlane(mv.x);
-        const unsigned hi32 = __builtin_amdgcn_readfirstlane(mv.y);
-        const unsigned long long mask = (static_cast<unsigned long long>(hi32) << 32) | lo32;
-        ++opened;
-        if (code & kBucketFlag) {
+        // The batch: up to eight cells from the top of the stack, as many as
+        // the stack has room to replace by their children (each cell nets at
+        // most seven more entries), and no bucket among them.  Group j reads
+        // entry sp-1-j (groups beyond the batch repeat its last entry).
+        const int kmax = min(min(8, sp), max((kStackCap - sp) / 7, 1));
+        const int e = sp - 1 - min(grp, kmax - 1);
+        const int vcell = stk_node[wave][e];
+        const uint2 vm = stk_mask[wave][e];
+        const unsigned long long bk = __builtin_amdgcn_ballot_w64((vcell & kBucketFlag) != 0);
+        int k = bk ? min(kmax, static_cast<int>(__builtin_ctzll(bk)) >> 3) : kmax;
+        if (k == 0) {
             // identical keys: every particle of the range, pair by pair
-            const int node = code & ~kBucketFlag;
+            --sp;
+            ++opened;
+            const int node = __builtin_amdgcn_readfirstlane(vcell) & ~kBucketFlag;
+            const unsigned lo32 = __builtin_amdgcn_readfirstlane(vm.x);
+            const unsigned hi32 = __builtin_amdgcn_readfirstlane(vm.y);
+            const unsigned long long mask = (static_cast<unsigned long long>(hi32) << 32) | lo32;
             const int f = node_first[node], l = node_last[node];
             for (int q = f; q <= l; ++q) {
                 if (cnt > kListCap - 4) flush();
@@ -1113,107 +1130,130 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
             }
             continue;
         }
-        if (cnt > kListCap - 8) flush();
-        // The whole block in scalar registers at once (the empty asm keeps the
-        // compiler from sinking each slot's loads into that slot's branch,
-        // which would wait on memory once per cache line), and child s's
-        // source and code in lane s for the list and stack writes.
-        const ChildBlock* B = blocks + code;
-        const float4 vsrc = B->src[lane & 7];
-        const int vcode = B->code[lane & 7];
-        const int4 c0 = *reinterpret_cast<const int4*>(B->code);
-        const int4 c1 = *reinterpret_cast<const int4*>(B->code + 4);
-        int codes[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-        float4 srcs[8];
-        float thrs[8];
-#pragma unroll
-        for (int slot = 0; slot < 8; ++slot) {
-            srcs[slot] = B->src[slot];
-            thrs[slot] = B->thr[slot];
+        // Up to eight list entries per cell: a smaller batch when at least
+        // four cells fit, else evaluate the list first.
+        if (cnt > kListCap - 8 * k) {
+            const int fit = (kListCap - cnt) >> 3;
+            if (fit >= 4) k = fit;
+            else flush();
         }
+        opened += static_cast<unsigned long long>(k);
+        const int base = sp - k;
+        // Lane 8j + s reads child s of cell j (lanes beyond the batch read the
+        // top cell's block and take no part), issued before the decisions.
+        const int top = __builtin_amdgcn_readfirstlane(vcell);  // all lanes active: group 0
+        const bool in_batch = grp < k;
+        const ChildBlock* VB = blocks + (in_batch ? vcell : top);
+        const float4 vsrc = VB->src[lane & 7];
+        const int vcode = VB->code[lane & 7];
+        // Decisions, cell by cell.  The whole block in scalar registers at
+        // once (the empty asm keeps the compiler from sinking each slot's
+        // loads into that slot's branch, which would wait on memory once per
+        // cache line).  A child particle needs no test: every lane of the
+        // mask but its own takes it.  A child cell is approximated by the
+        // lanes with !(r2 < thr) (the reference's side / |p - COM| < theta,
+        // see open_threshold) and opened by the others; that lane mask goes
+        // to lane 8j + s.  The children are packed, the first empty code
+        // ends them; a cell has at least two.
+        for (int j = 0; j < k; ++j) {
+            const int code = __builtin_amdgcn_readlane(vcell, j * 8);
+            const ChildBlock* B = blocks + code;
+            const int4 c0 = *reinterpret_cast<const int4*>(B->code);
+            const int4 c1 = *reinterpret_cast<const int4*>(B->code + 4);
+            int codes[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+            float4 srcs[8];
+            float thrs[8];
 #pragma unroll
-        for (int slot = 0; slot < 8; ++slot)
-            asm volatile("" ::"s"(srcs[slot].x), "s"(srcs[slot].y), "s"(srcs[slot].z), "s"(srcs[slot].w),
-                         "s"(thrs[slot]), "s"(codes[slot]));
-        // A child particle needs no test: every lane of the mask but its own
-        // takes it.  A child cell is approximated by the lanes with
-        // !(r2 < thr) (the reference's side / |p - COM| < theta, see
-        // open_threshold) and opened by the others; that lane mask goes to
-        // lane s.  The children are packed, the first empty code ends them;
-        // a cell has at least two.
-        auto decide = [&](auto S) {
-            constexpr int slot = decltype(S)::value;
-            if (codes[slot] < 0) return;
-            const float dx = me.x - srcs[slot].x, dy = me.y - srcs[slot].y, dz = me.z - srcs[slot].z;
-            const float r2 = (dx * dx + dy * dy) + dz * dz;
-            const unsigned long long ge = __builtin_amdgcn_ballot_w64(!(r2 < thrs[slot]));
-            write_lane64<slot>(ge_lo, ge_hi, ge);
-        };
-        using std::integral_constant;
-        decide(integral_constant<int, 0>{});
-        decide(integral_constant<int, 1>{});
-        if (codes[2] != kEmpty) {
-            decide(integral_constant<int, 2>{});
-            if (codes[3] != kEmpty) {
-                decide(integral_constant<int, 3>{});
-                if (codes[4] != kEmpty) {
-                    decide(integral_constant<int, 4>{});
-                    if (codes[5] != kEmpty) {
-                        decide(integral_constant<int, 5>{});
-                        if (codes[6] != kEmpty) {
-                            decide(integral_constant<int, 6>{});
-                            if (codes[7] != kEmpty) decide(integral_constant<int, 7>{});
+            for (int slot = 0; slot < 8; ++slot) {
+                srcs[slot] = B->src[slot];
+                thrs[slot] = B->thr[slot];
+            }
+#pragma unroll
+            for (int slot = 0; slot < 8; ++slot)
+                asm volatile("" ::"s"(srcs[slot].x), "s"(srcs[slot].y), "s"(srcs[slot].z), "s"(srcs[slot].w),
+                             "s"(thrs[slot]), "s"(codes[slot]));
+            const int lbase = j * 8;
+            auto decide = [&](auto S) {
+                constexpr int slot = decltype(S)::value;
+                if (codes[slot] < 0) return;
+                const float dx = me.x - srcs[slot].x, dy = me.y - srcs[slot].y, dz = me.z - srcs[slot].z;
+                const float r2 = (dx * dx + dy * dy) + dz * dz;
+                const unsigned long long ge = __builtin_amdgcn_ballot_w64(!(r2 < thrs[slot]));
+                write_lane64(ge_lo, ge_hi, ge, lbase + slot);
+            };
+            using std::integral_constant;
+            decide(integral_constant<int, 0>{});
+            decide(integral_constant<int, 1>{});
+            if (codes[2] != kEmpty) {
+                decide(integral_constant<int, 2>{});
+                if (codes[3] != kEmpty) {
+                    decide(integral_constant<int, 3>{});
+                    if (codes[4] != kEmpty) {
+                        decide(integral_constant<int, 4>{});
+                        if (codes[5] != kEmpty) {
+                            decide(integral_constant<int, 5>{});
+                            if (codes[6] != kEmpty) {
+                                decide(integral_constant<int, 6>{});
+                                if (codes[7] != kEmpty) decide(integral_constant<int, 7>{});
+                            }
                         }
                     }
                 }
             }
         }
-        // Lane s forms child s's two masks: approximating lanes am (for a
-        // particle every lane of the mask but the particle's own) and
-        // opening lanes om (none for a particle).
-        const bool child = lane < 8 && vcode != kEmpty;
-        const unsigned long long children = __builtin_amdgcn_ballot_w64(vcode != kEmpty) & 0xffull;
+        // Lane 8j + s forms child s's two masks: approximating lanes am (for a
+        // particle every lane of the mask but the particle's own) and opening
+        // lanes om (none for a particle).
+        const unsigned long long mask = (static_cast<unsigned long long>(vm.y) << 32) | vm.x;
+        const unsigned long long batch = k == 8 ? ~0ull : ((1ull << (8 * k)) - 1ull);
+        const bool child = in_batch && vcode != kEmpty;
+        const unsigned long long children = __builtin_amdgcn_ballot_w64(vcode != kEmpty) & batch;
         const bool leaf = vcode < 0;
         const unsigned own = static_cast<unsigned>(dec_leaf(vcode) - wave_base);
         const unsigned long long self = (leaf && own < 64u) ? (1ull << own) : 0ull;
         const unsigned long long ge = leaf ? ~0ull : ((static_cast<unsigned long long>(ge_hi) << 32) | ge_lo);
         const unsigned long long am = mask & ge & ~self;
         const unsigned long long om = mask & ~ge;
-        // List entries, in child order, written by the children's lanes.
+        // List entries, in lane order, written by the children's lanes.
         const bool wl = child && am != 0ull;
         const unsigned long long has_l = __builtin_amdgcn_ballot_w64(am != 0ull) & children;
         if (wl) {
-            const int at = static_cast<int>(__builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_l), cnt));
+            const int at = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                static_cast<unsigned>(has_l >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_l), cnt)));
             lst[wave][at].src = vsrc;
             lst[wave][at].mask = make_uint2(static_cast<unsigned>(am), static_cast<unsigned>(am >> 32));
             lane_count += __popcll(am);
         }
         cnt += __popcll(has_l);
-        // Opened cells onto the stack, in child order (the last is popped
-        // first); at most eight, so the room is checked once.
+        // Opened cells onto the stack in place of the batch, in reverse lane
+        // order: the top cell's first opened child ends on top.
         const bool ws = child && om != 0ull;  // a cell: a particle's ge is all lanes
         const unsigned long long has_s = __builtin_amdgcn_ballot_w64(om != 0ull) & children;
-        const int base = sp;
-        if (base <= kStackCap - 8) {
+        const int pushed = __popcll(has_s);
+        const int last = base + pushed - 1;
+        if (last < kStackCap) {
             if (ws) {
-                const int at = static_cast<int>(__builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_s), base));
+                const int at = last - static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                                          static_cast<unsigned>(has_s >> 32),
+                                          __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_s), 0u)));
                 stk_node[wave][at] = vcode;
                 stk_mask[wave][at] = make_uint2(static_cast<unsigned>(om), static_cast<unsigned>(om >> 32));
             }
-            sp = base + __popcll(has_s);
+            sp = base + pushed;
         } else {
+            // only a one-cell batch with less than seven entries of room:
+            // the entries beyond the capacity are lost and reported
             if (ws) {
-                const int at = static_cast<int>(__builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_s), base));
+                const int at = last - static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                                          static_cast<unsigned>(has_s >> 32),
+                                          __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_s), 0u)));
                 if (at < kStackCap) {
                     stk_node[wave][at] = vcode;
                     stk_mask[wave][at] = make_uint2(static_cast<unsigned>(om), static_cast<unsigned>(om >> 32));
                 }
             }
-            sp = base + __popcll(has_s);
-            if (sp > kStackCap) {
-                sp = kStackCap;
-                overflowed = true;
-            }
+            sp = kStackCap;
+            overflowed = true;
         }
     }
     if (overflowed && lane == 0) atomicOr(A.overflow, 1u);
