@@ -1044,9 +1044,9 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
     // 64-bit per-lane total at every flush
     unsigned lane_count = 0;
     unsigned long long lane_total = 0;
-    // during a pop, child cell s's mask of the lanes at or beyond its opening
-    // threshold, in lane s; lanes of particle children and lanes beyond the
-    // block's children hold stale values and are not used
+    // during a batch, child cell s of cell j's mask of the lanes at or beyond
+    // its opening threshold, in lane 8j + s; lanes of particle children and
+    // lanes beyond the batch's children are not used
     unsigned ge_lo = 0u, ge_hi = 0u;
 
     // Append a source held in scalar registers (written by lane 0).
@@ -1153,8 +1153,11 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         // mask but its own takes it.  A child cell is approximated by the
         // lanes with !(r2 < thr) (the reference's side / |p - COM| < theta,
         // see open_threshold) and opened by the others; that lane mask goes
-        // to lane 8j + s.  The children are packed, the first empty code
-        // ends them; a cell has at least two.
+        // to lane 8j + s, unless it is every lane (the lanes' masks start as
+        // all lanes).  The children are packed, the first empty code ends
+        // them; a cell has at least two.
+        ge_lo = ~0u;
+        ge_hi = ~0u;
         for (int j = 0; j < k; ++j) {
             const int code = __builtin_amdgcn_readlane(vcell, j * 8);
             const ChildBlock* B = blocks + code;
@@ -1179,7 +1182,7 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
                 const float dx = me.x - srcs[slot].x, dy = me.y - srcs[slot].y, dz = me.z - srcs[slot].z;
                 const float r2 = (dx * dx + dy * dy) + dz * dz;
                 const unsigned long long ge = __builtin_amdgcn_ballot_w64(!(r2 < thrs[slot]));
-                write_lane64(ge_lo, ge_hi, ge, lbase + slot);
+                if (ge != ~0ull) write_lane64(ge_lo, ge_hi, ge, lbase + slot);
             };
             using std::integral_constant;
             decide(integral_constant<int, 0>{});
