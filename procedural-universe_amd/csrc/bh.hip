@@ -614,45 +614,116 @@ __global__ __launch_bounds__(256) void block_kernel(BlockArgs A) {
     if (i >= A.m) return;
     const int self = A.info[i];
     if (!info_kept(self) || info_level(self) >= kBucketLevel) return;
-    ChildBlock& B = A.blocks[i];
-    int slot = 0;
-    // a child particle (c < 0), or a child cell with its word
-    auto emit = [&](int c, int info) {
-        if (slot >= 8) return;  // cannot happen (at most 2^3 children); keeps a bad tree in bounds
-        if (c < 0) {
-            B.src[slot] = A.pos_sorted[dec_leaf(c)];
-            B.thr[slot] = 0.f;
-            B.code[slot] = c;
-        } else {
-            const int lv = info_level(info);
-            B.src[slot] = A.node_com[c];
-            B.thr[slot] = A.thr[lv];
-            B.code[slot] = lv >= kBucketLevel ? (c | kBucketFlag) : c;
-        }
-        ++slot;
-    };
+    // The descent, one binary level at a time with every node's children and
+    // word loaded together (a node of the radix tree always has two children,
+    // so each load is in range), into fixed positions: the path (a, b, c)
+    // below the cell is position 4a + 2b + c, and a child found higher up
+    // takes the first position of its subtree.  Positions are key order.
+    int code[8], word[8];
+    bool found[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+        code[p] = kEmpty;
+        word[p] = 0;
+        found[p] = false;
+    }
     const int2 top = A.child[i];
+    const int c1[2] = {top.x, top.y};
+    int f1[2];
+    int2 mid[2];
+#pragma unroll
     for (int a = 0; a < 2; ++a) {
-        const int c1 = a ? top.y : top.x;
-        const int f1 = c1 < 0 ? 0 : A.info[c1];
-        if (c1 < 0 || info_kept(f1)) {
-            emit(c1, f1);
-            continue;
-        }
-        const int2 mid = A.child[c1];
-        for (int b = 0; b < 2; ++b) {
-            const int c2 = b ? mid.y : mid.x;
-            const int f2 = c2 < 0 ? 0 : A.info[c2];
-            if (c2 < 0 || info_kept(f2)) {
-                emit(c2, f2);
-                continue;
-            }
-            const int2 low = A.child[c2];
-            emit(low.x, low.x < 0 ? 0 : A.info[low.x]);
-            emit(low.y, low.y < 0 ? 0 : A.info[low.y]);
+        f1[a] = 0;
+        mid[a] = make_int2(kEmpty, kEmpty);
+        if (c1[a] >= 0) {
+            f1[a] = A.info[c1[a]];
+            mid[a] = A.child[c1[a]];
         }
     }
-    for (int s = slot; s < 8; ++s) B.code[s] = kEmpty;
+    bool open1[2], open2[2][2];
+    int f2[2][2];
+    int2 low[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        open1[a] = c1[a] >= 0 && !info_kept(f1[a]);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int c2 = b ? mid[a].y : mid[a].x;
+            f2[a][b] = 0;
+            low[a][b] = make_int2(kEmpty, kEmpty);
+            if (open1[a] && c2 >= 0) {
+                f2[a][b] = A.info[c2];
+                low[a][b] = A.child[c2];
+            }
+            open2[a][b] = open1[a] && c2 >= 0 && !info_kept(f2[a][b]);
+        }
+    }
+    int f3[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int c3 = c ? low[a][b].y : low[a][b].x;
+                f3[a][b][c] = (open2[a][b] && c3 >= 0) ? A.info[c3] : 0;
+            }
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        if (!open1[a]) {
+            code[4 * a] = c1[a];
+            word[4 * a] = f1[a];
+            found[4 * a] = true;
+            continue;
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int p = 4 * a + 2 * b;
+            if (!open2[a][b]) {
+                code[p] = b ? mid[a].y : mid[a].x;
+                word[p] = f2[a][b];
+                found[p] = true;
+                continue;
+            }
+            code[p] = low[a][b].x;
+            word[p] = f3[a][b][0];
+            found[p] = true;
+            code[p + 1] = low[a][b].y;
+            word[p + 1] = f3[a][b][1];
+            found[p + 1] = true;
+        }
+    }
+    // Every child's record loaded before any is written (the block may not
+    // alias the sources, but the compiler cannot know it).
+    float4 src[8];
+    float th[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+        src[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        th[p] = 0.f;
+        if (found[p]) {
+            const int c = code[p];
+            const float4* at = c < 0 ? A.pos_sorted + dec_leaf(c) : A.node_com + c;
+            src[p] = *at;
+            if (c >= 0) {
+                const int lv = info_level(word[p]);
+                th[p] = A.thr[lv];
+                if (lv >= kBucketLevel) code[p] = c | kBucketFlag;
+            }
+        }
+    }
+    ChildBlock& B = A.blocks[i];
+    int slot = 0;
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+        if (found[p]) {
+            B.src[slot] = src[p];
+            B.thr[slot] = th[p];
+            B.code[slot] = code[p];
+            ++slot;
+        }
+    }
+    for (int s2 = slot; s2 < 8; ++s2) B.code[s2] = kEmpty;
 }
 
 // ---- 5. FP64 mass and centre of mass ------------------------------------------------
@@ -963,16 +1034,17 @@ __device__ __forceinline__ float lane_gm(float gm, unsigned w, int lane) {
 
 // (lo, hi) = x in lane l only (register writes; the other lanes keep lo and
 // hi).  The lane number goes through M0: one instruction may read only one
-// SGPR besides M0.  The compiler does not look inside the asm, so the asm
-// waits for the hazards its operands can carry: x is usually a compare result
-// (a VALU write of VCC read by a VALU instruction as a scalar operand needs
-// one wait state, of another SGPR pair two), and a lane select written just
-// before needs up to four.
+// SGPR besides M0.  M0 is reserved to the compiler, which sets it right before
+// each instruction that reads it; no instruction of the traversal kernels
+// does, so the asm may overwrite it.  The compiler does not look inside the
+// asm, so the asm waits for the hazards its operands can carry: x is usually a
+// compare result (a VALU write of VCC read by a VALU instruction as a scalar
+// operand needs one wait state, of another SGPR pair two), and a lane select
+// written just before needs up to four.
 __device__ __forceinline__ void write_lane64(unsigned& lo, unsigned& hi, unsigned long long x, int l) {
     asm("s_mov_b32 m0, %4\n\ts_nop 4\n\tv_writelane_b32 %0, %2, m0\n\tv_writelane_b32 %1, %3, m0"
         : "+v"(lo), "+v"(hi)
-        : "s"(static_cast<unsigned>(x)), "s"(static_cast<unsigned>(x >> 32)), "s"(l)
-        : "m0");
+        : "s"(static_cast<unsigned>(x)), "s"(static_cast<unsigned>(x >> 32)), "s"(l));
 }
 
 template <int LAW>
