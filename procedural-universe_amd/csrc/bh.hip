@@ -31,7 +31,8 @@
 //                    (at most eight, in key order): sources, opening
 //                    thresholds and codes, all the traversal reads of it;
 //  6. traversal      one lane per particle in Morton order; a wave walks the
-//                    octree with one stack in LDS of (cell, lanes) entries.
+//                    octree with one stack in LDS of (cell, lanes) entries,
+//                    up to eight cells per step of the walk.
 //                    Each lane makes the reference's own decision at each cell
 //                    (Octree::CalculateForce, Octree.cpp:107-145): approximate
 //                    with the cell's monopole iff side / |p - COM| < theta,
@@ -1006,16 +1007,17 @@ struct TravArgs {
 using core::kahan_add;
 
 // The walk and the arithmetic are separated.  The walk (uniform over the
-// wave) pops a cell that some lanes opened, reads its child block, and decides
-// per lane and per child: a child cell is approximated (its monopole goes to
-// the list with the lanes that approximate it) or opened (pushed with the
-// lanes that open it); a child particle goes to the list with every lane that
-// opened the parent but its own.  When the list fills, every lane evaluates
-// all entries in a tight loop - four sources at a time with grouped rsq, the
-// source's G*m zeroed for lanes outside the entry - and adds the result to its
-// compensated sum.  A child's entry is written by the lane that holds the
-// child's record in vector registers (one vector load of the block per pop),
-// so an append moves only the address and the lane mask.
+// wave) pops a batch of up to eight cells that some lanes opened, reads their
+// child blocks, and decides per lane and per child: a child cell is
+// approximated (its monopole goes to the list with the lanes that approximate
+// it) or opened (pushed with the lanes that open it); a child particle goes to
+// the list with every lane that opened the parent but its own.  When the list
+// fills, every lane evaluates all entries in a tight loop - four sources at a
+// time with grouped rsq, the source's G*m zeroed for lanes outside the entry -
+// and adds the result to its compensated sum.  A child's entry is written by
+// the lane that holds the child's record in vector registers (lane 8j + s for
+// child s of the batch's cell j, one vector load per batch), so an append
+// moves only the address and the lane mask.
 constexpr int kListCap = 96;
 
 // A list entry: the source (x, y, z, G m) and the lanes it applies to.
@@ -1112,7 +1114,7 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
     int cnt = 0;
     int sp = 0;
     bool overflowed = false;
-    // interactions of the entries this lane wrote (lanes 0-7), folded into a
+    // interactions of the entries this lane wrote, folded into a
     // 64-bit per-lane total at every flush
     unsigned lane_count = 0;
     unsigned long long lane_total = 0;
