@@ -590,8 +590,9 @@ __device__ void radix_tree_block(const TreeArgs& T, int block) {
 }
 
 // Every octree cell below the bucket level writes its own child block: its
-// children in key order (which is octant order) packed into the first slots,
-// the remaining codes kEmpty.  A child cell gets (COM, G M), its opening
+// child cells, then its child particles, each in key order (which is octant
+// order), packed into the first slots, the remaining codes kEmpty.  A child
+// cell gets (COM, G M), its opening
 // threshold and its node; a particle (position, G m), threshold 0 (r2 >= 0
 // always holds) and its code.  The children are the first kept nodes and the
 // particles below the cell in the radix tree.  A node that is not kept has
@@ -713,11 +714,22 @@ __global__ __launch_bounds__(256) void block_kernel(BlockArgs A) {
             }
         }
     }
+    // The child cells first, then the particles, each in key order: the walk
+    // tests the cells and stops at the first code below zero.
     ChildBlock& B = A.blocks[i];
     int slot = 0;
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
-        if (found[p]) {
+        if (found[p] && code[p] >= 0) {
+            B.src[slot] = src[p];
+            B.thr[slot] = th[p];
+            B.code[slot] = code[p];
+            ++slot;
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+        if (found[p] && code[p] < 0) {
             B.src[slot] = src[p];
             B.thr[slot] = th[p];
             B.code[slot] = code[p];
@@ -1228,8 +1240,9 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         // lanes with !(r2 < thr) (the reference's side / |p - COM| < theta,
         // see open_threshold) and opened by the others; that lane mask goes
         // to lane 8j + s, unless it is every lane (the lanes' masks start as
-        // all lanes).  The children are packed, the first empty code ends
-        // them; a cell has at least two.
+        // all lanes).  A block holds its child cells first, then its
+        // particles, then empty codes, so the tests stop at the first code
+        // below zero.
         ge_lo = ~0u;
         ge_hi = ~0u;
         for (int j = 0; j < k; ++j) {
@@ -1252,26 +1265,29 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
             const int lbase = j * 8;
             auto decide = [&](auto S) {
                 constexpr int slot = decltype(S)::value;
-                if (codes[slot] < 0) return;
                 const float dx = me.x - srcs[slot].x, dy = me.y - srcs[slot].y, dz = me.z - srcs[slot].z;
                 const float r2 = (dx * dx + dy * dy) + dz * dz;
                 const unsigned long long ge = __builtin_amdgcn_ballot_w64(!(r2 < thrs[slot]));
                 if (ge != ~0ull) write_lane64(ge_lo, ge_hi, ge, lbase + slot);
             };
             using std::integral_constant;
-            decide(integral_constant<int, 0>{});
-            decide(integral_constant<int, 1>{});
-            if (codes[2] != kEmpty) {
-                decide(integral_constant<int, 2>{});
-                if (codes[3] != kEmpty) {
-                    decide(integral_constant<int, 3>{});
-                    if (codes[4] != kEmpty) {
-                        decide(integral_constant<int, 4>{});
-                        if (codes[5] != kEmpty) {
-                            decide(integral_constant<int, 5>{});
-                            if (codes[6] != kEmpty) {
-                                decide(integral_constant<int, 6>{});
-                                if (codes[7] != kEmpty) decide(integral_constant<int, 7>{});
+            if (codes[0] >= 0) {
+                decide(integral_constant<int, 0>{});
+                if (codes[1] >= 0) {
+                    decide(integral_constant<int, 1>{});
+                    if (codes[2] >= 0) {
+                        decide(integral_constant<int, 2>{});
+                        if (codes[3] >= 0) {
+                            decide(integral_constant<int, 3>{});
+                            if (codes[4] >= 0) {
+                                decide(integral_constant<int, 4>{});
+                                if (codes[5] >= 0) {
+                                    decide(integral_constant<int, 5>{});
+                                    if (codes[6] >= 0) {
+                                        decide(integral_constant<int, 6>{});
+                                        if (codes[7] >= 0) decide(integral_constant<int, 7>{});
+                                    }
+                                }
                             }
                         }
                     }
