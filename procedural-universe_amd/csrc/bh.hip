@@ -73,10 +73,14 @@ constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kTravThreads = 64;  // one wavefront per workgroup: the finest tail
 constexpr int kTravWaves = kTravThreads / 64;
-// Pending (cell, lanes) entries per wave.  A batch of k cells replaces them by
-// at most 8k children, so a batch is only as large as the room left allows;
-// a single cell beyond that goes through a checked push that reports overflow.
-constexpr int kStackCap = 224;
+// Pending (cell, lanes) entries per wave.  Popping one cell at a time (a
+// depth-first walk) needs at most 7 * 21 + 1 entries above the cell it starts
+// from: seven pending siblings per level below it and the last cell's eight
+// children.  A batch of k cells replaces them by at most 8k children, so a
+// batch is only as large as keeps that reserve free afterwards, and the stack
+// cannot overflow (the checked push that reports overflow is a safeguard).
+constexpr int kStackCap = 288;
+constexpr int kDfsReserve = 7 * 22;
 constexpr int kEmpty = -1;
 
 // Child encoding: >= 0 internal node; <= -2 particle at sorted position (-c - 2).
@@ -1032,13 +1036,6 @@ using core::kahan_add;
 // moves only the address and the lane mask.
 constexpr int kListCap = 96;
 
-// A list entry: the source (x, y, z, G m) and the lanes it applies to.
-struct ListEntry {
-    float4 src;
-    uint2 mask;
-    uint2 pad;
-};
-
 // G*m where the lane's bit of the entry mask is set, else 0 (the bit
 // sign-extended to a full-width and-mask); w is the mask word holding the
 // lane's bit (lanes 0-31 read the low word, lanes 32-63 the high one).
@@ -1061,20 +1058,22 @@ __device__ __forceinline__ void write_lane64(unsigned& lo, unsigned& hi, unsigne
         : "s"(static_cast<unsigned>(x)), "s"(static_cast<unsigned>(x >> 32)), "s"(l));
 }
 
+// A list entry is the source (x, y, z, G m) in one array and the lanes it
+// applies to in another.
 template <int LAW>
-__device__ __forceinline__ void flush_list(const ListEntry* lst, int cnt, int lane, float px,
+__device__ __forceinline__ void flush_list(const float4* lsrc, const uint2* lmask, int cnt, int lane, float px,
                                            float py, float pz, float soft, float tiny, float& sx, float& sy,
                                            float& sz, float& cx, float& cy, float& cz) {
     float ax = 0.f, ay = 0.f, az = 0.f;
-    // the lane's word of every entry's mask (entries are 8 words apart)
-    const unsigned* words = reinterpret_cast<const unsigned*>(&lst[0].mask) + (lane >> 5);
+    // the lane's word of every entry's mask
+    const unsigned* words = reinterpret_cast<const unsigned*>(lmask) + (lane >> 5);
     for (int e = 0; e < cnt; e += 4) {
         float4 src[4];
         float dx[4], dy[4], dz[4], q[4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            src[g] = lst[e + g].src;
-            src[g].w = lane_gm(src[g].w, words[(e + g) * 8], lane);
+            src[g] = lsrc[e + g];
+            src[g].w = lane_gm(src[g].w, words[(e + g) * 2], lane);
             q[g] = core::pair_arg<LAW>(src[g], px, py, pz, soft, tiny, dx[g], dy[g], dz[g]);
         }
         core::rsq4(q[0], q[1], q[2], q[3]);
@@ -1109,7 +1108,8 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
                                                                 const int* __restrict__ node_last, TravArgs A) {
     __shared__ int stk_node[kTravWaves][kStackCap];
     __shared__ uint2 stk_mask[kTravWaves][kStackCap];
-    __shared__ ListEntry lst[kTravWaves][kListCap];
+    __shared__ float4 lsrc[kTravWaves][kListCap];
+    __shared__ uint2 lmask[kTravWaves][kListCap];
     __shared__ unsigned long long red[3][kTravWaves];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: list and stack addresses in SGPRs
@@ -1138,8 +1138,8 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
     // Append a source held in scalar registers (written by lane 0).
     auto append = [&](const float4 src, unsigned long long mask) {
         if (lane == 0) {
-            lst[wave][cnt].src = src;
-            lst[wave][cnt].mask = make_uint2(static_cast<unsigned>(mask), static_cast<unsigned>(mask >> 32));
+            lsrc[wave][cnt] = src;
+            lmask[wave][cnt] = make_uint2(static_cast<unsigned>(mask), static_cast<unsigned>(mask >> 32));
         }
         count += static_cast<unsigned long long>(__popcll(mask));
         ++cnt;
@@ -1148,12 +1148,13 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         entries += static_cast<unsigned long long>(cnt);
         const int pad = (4 - (cnt & 3)) & 3;
         if (lane < pad) {
-            lst[wave][cnt + lane].src = make_float4(0.f, 0.f, 0.f, 0.f);
-            lst[wave][cnt + lane].mask = make_uint2(0u, 0u);
+            lsrc[wave][cnt + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+            lmask[wave][cnt + lane] = make_uint2(0u, 0u);
         }
         const int total = __builtin_amdgcn_readfirstlane(cnt + pad);
         __builtin_amdgcn_wave_barrier();
-        flush_list<LAW>(lst[wave], total, lane, me.x, me.y, me.z, soft, tiny, sx, sy, sz, cx, cy, cz);
+        flush_list<LAW>(lsrc[wave], lmask[wave], total, lane, me.x, me.y, me.z, soft, tiny, sx, sy, sz, cx, cy,
+                        cz);
         __builtin_amdgcn_wave_barrier();
         cnt = 0;
         lane_total += lane_count;
@@ -1192,9 +1193,10 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         __builtin_amdgcn_wave_barrier();
         // The batch: up to eight cells from the top of the stack, as many as
         // the stack has room to replace by their children (each cell nets at
-        // most seven more entries), and no bucket among them.  Group j reads
-        // entry sp-1-j (groups beyond the batch repeat its last entry).
-        const int kmax = min(min(8, sp), max((kStackCap - sp) / 7, 1));
+        // most seven more entries) while keeping the depth-first reserve, at
+        // least one, and no bucket among them.  Group j reads entry sp-1-j
+        // (groups beyond the batch repeat its last entry).
+        const int kmax = min(min(8, sp), max((kStackCap - kDfsReserve - sp) / 7, 1));
         const int e = sp - 1 - min(grp, kmax - 1);
         const int vcell = stk_node[wave][e];
         const uint2 vm = stk_mask[wave][e];
@@ -1313,8 +1315,8 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         if (wl) {
             const int at = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
                 static_cast<unsigned>(has_l >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_l), cnt)));
-            lst[wave][at].src = vsrc;
-            lst[wave][at].mask = make_uint2(static_cast<unsigned>(am), static_cast<unsigned>(am >> 32));
+            lsrc[wave][at] = vsrc;
+            lmask[wave][at] = make_uint2(static_cast<unsigned>(am), static_cast<unsigned>(am >> 32));
             lane_count += __popcll(am);
         }
         cnt += __popcll(has_l);
@@ -1334,8 +1336,8 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
             }
             sp = base + pushed;
         } else {
-            // only a one-cell batch with less than seven entries of room:
-            // the entries beyond the capacity are lost and reported
+            // not reached (the depth-first reserve): the entries beyond the
+            // capacity would be lost and reported
             if (ws) {
                 const int at = last - static_cast<int>(__builtin_amdgcn_mbcnt_hi(
                                           static_cast<unsigned>(has_s >> 32),

@@ -121,6 +121,35 @@ def test_theta_zero_is_direct_sum(gpu):
     assert rel_err_rows(a_bh, a_d).max() < 1e-5  # two FP32 sums of the same pairs in different orders
 
 
+def test_theta_zero_full_octree_keeps_every_pair(gpu):
+    """theta = 0 opens every cell for every lane.  Two particles in each of the
+    64^3 level-7 cells of [0, 4000)^3 under the reference's fixed root make
+    every cell of six levels have eight child cells, the deepest stack the walk
+    can meet: batches of eight cells with no room kept for a depth-first
+    descent would stack 8, 64, 120, 176, 218 entries and then overflow 224.
+    Every pair is evaluated exactly once and the stack never overflows."""
+    pu = gpu
+    h = 62.5  # side of a level-7 cell of the 8000-wide root
+    g = np.arange(64, dtype=np.float64) * h
+    corner = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pos = np.concatenate([corner + 0.25 * h, corner + 0.75 * h]).astype(np.float32)
+    n = len(pos)
+    vel = np.zeros_like(pos)
+    mass = np.full(n, 1e25)
+    cfg = pu.reference_config(pu.METHOD_BARNES_HUT, theta=0.0)
+    with _bh(pu, cfg, pos, vel, mass) as sim:
+        a_bh = sim.accelerations()
+        st = sim.bh_stats()
+    with _bh(pu, pu.reference_config(pu.METHOD_DIRECT), pos, vel, mass) as sim:
+        a_d = sim.accelerations()
+    print(f"theta 0 lattice, N={n}: stack overflow {st['stack_overflow']}, interactions {st['interactions']}")
+    assert st["stack_overflow"] == 0
+    assert st["interactions"] == n * (n - 1)
+    # the lattice's symmetry cancels most of each sum: compare against the RMS
+    scale = np.sqrt((a_d.astype(np.float64) ** 2).sum(1).mean())
+    assert np.abs(a_bh.astype(np.float64) - a_d).max() < 1e-5 * scale
+
+
 def test_plummer_law_against_fp64(gpu, oracle):
     """Law P (G m r / (r^2 + eps^2)^(3/2)) through the octree: theta 0 is the
     FP64 direct sum to FP32 rounding, theta 0.5 converges towards it."""
