@@ -38,7 +38,7 @@ DIRECT_1M_TRAFFIC_BYTES = 2.1854e9
 # (profiles/bh_traverse_n4m_theta05_traffic.md) and the 512-galaxy batch step
 # (profiles/batch_step_512_galaxies_traffic.md); they hold for the default
 # workloads only.
-BH_4M_TRAFFIC_BYTES = 3.5820e+09
+BH_4M_TRAFFIC_BYTES = 3.5716e+09
 BATCH_512_TRAFFIC_BYTES = 1.4884e9
 FLOP_PER_INTERACTION = 20  # Nyland, Harris & Prins (GPU Gems 3, ch. 31) convention
 METRIC = "interactions/s & % FP32 VALU peak at N=1M; batched p50/p99 step latency"
