@@ -282,6 +282,7 @@ __global__ __launch_bounds__(256) void keys_kernel(const float4* __restrict__ po
 
 // ---- 3. radix sort -----------------------------------------------------------
 
+// Digit counts of each tile, digit-major (hist[d * nblocks + tile]).
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const unsigned long long* __restrict__ keys, int n,
                                                                   int shift, unsigned* __restrict__ hist,
                                                                   int nblocks) {
@@ -297,12 +298,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const unsigned
     hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of `total` counters: chunks of kScanChunk are reduced in
-// parallel (coalesced), the chunk sums are scanned by one workgroup, then each
-// chunk is scanned and offset.
 constexpr int kScanThreads2 = 256;
 constexpr int kScanPer = 16;
-constexpr int kScanChunk = kScanThreads2 * kScanPer;
 
 __device__ inline unsigned block_exclusive_scan_u32(unsigned v, unsigned* sh, unsigned* total) {
     const int t = threadIdx.x;
@@ -320,50 +317,34 @@ __device__ inline unsigned block_exclusive_scan_u32(unsigned v, unsigned* sh, un
     return incl - v;
 }
 
-__global__ __launch_bounds__(kScanThreads2) void scan_reduce_kernel(const unsigned* __restrict__ in, int total,
-                                                                    unsigned* __restrict__ chunk_sums) {
+// Workgroup d replaces digit d's row of tile counts by its exclusive scan
+// (the tile's offset among the keys with digit d) and writes the row's total
+// to totals[d]; the scatter adds the totals of the smaller digits.
+__global__ __launch_bounds__(kScanThreads2) void radix_rowscan_kernel(unsigned* __restrict__ hist, int nblocks,
+                                                                      unsigned* __restrict__ totals) {
     __shared__ unsigned sh[kScanThreads2];
-    const int base = blockIdx.x * kScanChunk;
-    unsigned s = 0;
-    for (int k = 0; k < kScanPer; ++k) {
-        const int i = base + k * kScanThreads2 + threadIdx.x;
-        if (i < total) s += in[i];
-    }
-    unsigned tot = 0;
-    block_exclusive_scan_u32(s, sh, &tot);
-    if (threadIdx.x == 0) chunk_sums[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(kScanThreads2) void scan_chunks_kernel(unsigned* __restrict__ chunk_sums, int nchunks) {
-    __shared__ unsigned sh[kScanThreads2];
+    const int d = blockIdx.x, t = threadIdx.x;
+    unsigned* row = hist + static_cast<size_t>(d) * nblocks;
     unsigned carry = 0;
-    for (int b = 0; b < nchunks; b += kScanThreads2) {
-        const int i = b + threadIdx.x;
-        const unsigned v = i < nchunks ? chunk_sums[i] : 0u;
-        unsigned tot = 0;
-        const unsigned ex = block_exclusive_scan_u32(v, sh, &tot);
-        if (i < nchunks) chunk_sums[i] = carry + ex;
-        carry += tot;
+    for (int base = 0; base < nblocks; base += kScanThreads2 * kScanPer) {
+        const int at = base + t * kScanPer;  // a contiguous run per thread
+        unsigned v[kScanPer];
+        unsigned sum = 0;
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            v[k] = at + k < nblocks ? row[at + k] : 0u;
+            sum += v[k];
+        }
+        unsigned chunk = 0;
+        unsigned run = carry + block_exclusive_scan_u32(sum, sh, &chunk);
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            if (at + k < nblocks) row[at + k] = run;
+            run += v[k];
+        }
+        carry += chunk;
     }
-}
-
-__global__ __launch_bounds__(kScanThreads2) void scan_apply_kernel(unsigned* __restrict__ data, int total,
-                                                                   const unsigned* __restrict__ chunk_off) {
-    __shared__ unsigned sh[kScanThreads2];
-    const int base = blockIdx.x * kScanChunk + threadIdx.x * kScanPer;  // contiguous run per thread
-    unsigned v[kScanPer];
-    unsigned s = 0;
-    for (int k = 0; k < kScanPer; ++k) {
-        const int i = base + k;
-        v[k] = i < total ? data[i] : 0u;
-        s += v[k];
-    }
-    unsigned run = chunk_off[blockIdx.x] + block_exclusive_scan_u32(s, sh, nullptr);
-    for (int k = 0; k < kScanPer; ++k) {
-        const int i = base + k;
-        if (i < total) data[i] = run;
-        run += v[k];
-    }
+    if (t == 0) totals[d] = carry;
 }
 
 // Counters: walk[3] (interactions, list entries, cells opened) and
@@ -411,8 +392,8 @@ template <bool FUSED, bool GATHER>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const unsigned long long* __restrict__ keys_in, const unsigned* __restrict__ idx_in,
     unsigned long long* __restrict__ keys_out, unsigned* __restrict__ idx_out, int n, int shift,
-    const unsigned* __restrict__ counts, int nblocks, const float4* __restrict__ pos_in,
-    float4* __restrict__ pos_out) {
+    const unsigned* __restrict__ counts, const unsigned* __restrict__ totals, int nblocks,
+    const float4* __restrict__ pos_in, float4* __restrict__ pos_out) {
     static_assert(kSortPerWave == 64 * kSortItems, "a wavefront's share is kSortItems rounds of 64 keys");
     __shared__ unsigned cnt[kSortWaves][256];  // per-wave digit counts, then per-wave offsets in the run
     __shared__ unsigned run_start[256];        // the tile's digit runs
@@ -460,8 +441,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
             cnt[w][tid] = run;
             run += c;
         }
-        // digit tid's place in the output: the exclusive prefix of the
-        // counts (given, or formed here from the raw counts)
+        // digit tid's place in the output: the digit's start (the totals of
+        // the smaller digits) plus the tile's offset among the digit's keys,
+        // both formed here from the raw counts or given by the row scan
         unsigned before = 0, total = 0;
         if constexpr (FUSED) {
             const unsigned* row = counts + static_cast<size_t>(tid) * nblocks;
@@ -472,6 +454,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
             }
         } else {
             before = counts[tid * nblocks + blockIdx.x];
+            total = totals[tid];
         }
         unsigned x = run, y = total;
 #pragma unroll
@@ -494,7 +477,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
             digit_start += wave_sum[1][w];
         }
         run_start[tid] = start;
-        dst_base[tid] = FUSED ? digit_start + before : before;
+        dst_base[tid] = digit_start + before;
     }
     __syncthreads();
 #pragma unroll
@@ -1403,7 +1386,7 @@ struct BHState {
     int* last = nullptr;
     int* info = nullptr;    // level | kKeptBit
     ChildBlock* blocks = nullptr;
-    unsigned* chunk_sums = nullptr;  // sort scan
+    unsigned* digit_totals = nullptr;  // sort: keys per digit of the current pass
     size_t chunk_elems = 0;
     DD* prefix = nullptr;            // [4][n + 1] double-double prefix sums
     DD* mass_chunk = nullptr;        // [4][nchunks]
@@ -1441,7 +1424,7 @@ void bh_release(BHState* b) {
     bh_free(b->last);
     bh_free(b->info);
     bh_free(b->blocks);
-    bh_free(b->chunk_sums);
+    bh_free(b->digit_totals);
     bh_free(b->prefix);
     bh_free(b->mass_chunk);
     bh_free(b->com);
@@ -1564,7 +1547,7 @@ nbody_status bh_resize(BHState* b, size_t n, nbody_sim* s) {
     NB_TRY(bh_alloc(b->last, m));
     NB_TRY(bh_alloc(b->info, m));
     NB_TRY(bh_alloc(b->blocks, m));
-    NB_TRY(bh_alloc(b->chunk_sums, static_cast<size_t>(blocks_for(256 * nb, kScanChunk))));
+    NB_TRY(bh_alloc(b->digit_totals, 256));
     NB_TRY(bh_alloc(b->prefix, 4 * (n + 1)));
     NB_TRY(bh_alloc(b->mass_chunk, 4 * static_cast<size_t>(blocks_for(n + 1, kMassChunk))));
     NB_TRY(bh_alloc(b->com, m));
@@ -1596,21 +1579,18 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     int cur = 0;
     for (int pass = 0; pass < 8; ++pass) {
         const int shift = 8 * pass;
-        hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], n, shift, b->hist, nb);
-        const int total = 256 * nb, nchunks = blocks_for(total, kScanChunk);
+        // Above kFusedScanTiles tiles the tile offsets within each digit and
+        // the digits' totals come from a separate scan of the counts.
         const bool fused = nb <= kFusedScanTiles;
-        if (!fused) {
-            hipLaunchKernelGGL(scan_reduce_kernel, dim3(nchunks), dim3(kScanThreads2), 0, st, b->hist, total,
-                               b->chunk_sums);
-            hipLaunchKernelGGL(scan_chunks_kernel, dim3(1), dim3(kScanThreads2), 0, st, b->chunk_sums, nchunks);
-            hipLaunchKernelGGL(scan_apply_kernel, dim3(nchunks), dim3(kScanThreads2), 0, st, b->hist, total,
-                               b->chunk_sums);
-        }
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], n, shift, b->hist, nb);
+        if (!fused)
+            hipLaunchKernelGGL(radix_rowscan_kernel, dim3(256), dim3(kScanThreads2), 0, st, b->hist, nb,
+                               b->digit_totals);
         // the last pass also writes the sorted positions
         auto scatter = pass < 7 ? (fused ? radix_scatter_kernel<true, false> : radix_scatter_kernel<false, false>)
                                 : (fused ? radix_scatter_kernel<true, true> : radix_scatter_kernel<false, true>);
         hipLaunchKernelGGL(scatter, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], b->idx[cur], b->keys[cur ^ 1],
-                           b->idx[cur ^ 1], n, shift, b->hist, nb, pos, b->pos_sorted);
+                           b->idx[cur ^ 1], n, shift, b->hist, b->digit_totals, nb, pos, b->pos_sorted);
         cur ^= 1;
     }
     b->sorted_buf = cur;
