@@ -877,10 +877,10 @@ constexpr int kMassFusedChunks = 256;
 template <bool FUSED>
 __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
     __shared__ DD wsum[4][kMassWaves];
-    __shared__ DD carry_sh[4];
+    __shared__ DD wpre[4][kMassWaves];  // the row's sum before each wave, carry included
+    __shared__ DD carry_sh[4];          // the sum before the row
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t stride = static_cast<size_t>(A.n) + 1;
-    DD carry[4];
     if constexpr (FUSED) {
         if (wave == 0) {
 #pragma unroll
@@ -891,12 +891,8 @@ __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
                 if (lane == 63) carry_sh[q] = incl;
             }
         }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 4; ++q) carry[q] = carry_sh[q];
     } else {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) carry[q] = A.chunk[q * A.nchunks + blockIdx.x];
+        if (threadIdx.x < 4) carry_sh[threadIdx.x] = A.chunk[threadIdx.x * A.nchunks + blockIdx.x];
     }
     for (int r = 0; r < kMassPer; ++r) {
         const int i = blockIdx.x * kMassChunk + r * kMassThreads + threadIdx.x;
@@ -909,15 +905,23 @@ __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
             if (lane == 63) wsum[q][wave] = incl[q];
         }
         __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            DD before = carry[q];
-            for (int w = 0; w < wave; ++w) before = dd_add(before, wsum[q][w]);
-            const DD ex = wave_exclusive(incl[q], lane);
-            if (i <= A.n) A.prefix[q * stride + i] = dd_add(before, ex);
-            for (int w = 0; w < kMassWaves; ++w) carry[q] = dd_add(carry[q], wsum[q][w]);
+        // thread q forms quantity q's sums before each wave and the next
+        // carry, adding in the same order as every thread would
+        if (threadIdx.x < 4) {
+            const int q = threadIdx.x;
+            DD acc = carry_sh[q];
+            for (int w = 0; w < kMassWaves; ++w) {
+                wpre[q][w] = acc;
+                acc = dd_add(acc, wsum[q][w]);
+            }
+            carry_sh[q] = acc;
         }
         __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const DD ex = wave_exclusive(incl[q], lane);
+            if (i <= A.n) A.prefix[q * stride + i] = dd_add(wpre[q][wave], ex);
+        }
     }
 }
 
