@@ -363,7 +363,7 @@ def test_cell_export_describes_the_octree(gpu):
 @pytest.mark.parametrize("n", [524288, 524289, 1048575, 1048576])
 def test_sort_and_mass_paths_at_their_thresholds(gpu, oracle, n):
     """The sort's scatter scans its own digit counts up to 256 tiles of 2048
-    keys (N <= 524,288) and uses separate scan passes above; the mass prefix
+    keys (N <= 524,288) and uses a separate scan of the counts above; the mass prefix
     folds its chunk scan up to 256 chunks of 4096 (N + 1 <= 1,048,576).  On
     both sides of both thresholds the keys and the stable order are bit-exact
     and the cells' FP64 masses are the sums of their particles' masses."""
@@ -389,6 +389,31 @@ def test_sort_and_mass_paths_at_their_thresholds(gpu, oracle, n):
     for c in cells[:: max(1, cells.size // 500)]:
         f, k = int(c["first"]), int(c["count"])
         assert c["mass"] == pytest.approx(m[f:f + k].sum(), rel=1e-12, abs=1e-15 * csum[-1])
+
+
+def test_sort_duplicate_heavy_above_tile_threshold(gpu, oracle):
+    """600,000 particles on 1,000 distinct points (about 600 identical keys
+    each) sort in 293 tiles, through the separate scan of the tile counts:
+    the keys and the stable order are bit-exact, and the buckets' coincident
+    particles give finite accelerations."""
+    pu = gpu
+    rng = np.random.default_rng(0x5EED0400)
+    pts = rng.uniform(-1.0, 1.0, size=(1000, 3)).astype(np.float32)
+    pos = np.ascontiguousarray(pts[rng.integers(0, 1000, 600000)])
+    vel = np.zeros_like(pos)
+    mass = rng.uniform(1.0, 2.0, len(pos))
+    cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=1e-4, theta=0.5)
+    with _bh(pu, cfg, pos, vel, mass) as sim:
+        a = sim.accelerations()
+        keys, order = sim.bh_tree_order()
+        st = sim.bh_stats()
+    lo, size = oracle.dynamic_root(pos)
+    ref_keys, outside = oracle.morton_keys(pos, lo, size)
+    ref_order = oracle.sort_order(ref_keys)
+    assert outside == 0 and st["stack_overflow"] == 0
+    assert np.array_equal(order, ref_order)
+    assert np.array_equal(keys, ref_keys[ref_order])
+    assert np.all(np.isfinite(a))
 
 
 def test_full_size_encounter_keys_and_tree(gpu, oracle):
