@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256) void bounds_kernel(const float4* __restrict__ 
     }
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w)
+        if (static_cast<int>(threadIdx.x) < w)
             for (int a = 0; a < 3; ++a) {
                 red[a][threadIdx.x] = fminf(red[a][threadIdx.x], red[a][threadIdx.x + w]);
                 red[3 + a][threadIdx.x] = fmaxf(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + w]);
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void dynamic_root_kernel(const float* __restri
     for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = pm[c];
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w)
+        if (static_cast<int>(threadIdx.x) < w)
             for (int c = 0; c < 6; ++c) red[c][threadIdx.x] = fminf(red[c][threadIdx.x], red[c][threadIdx.x + w]);
         __syncthreads();
     }
