@@ -596,7 +596,7 @@ def main():
             "max_rel": float(err.max()),
             "oracle_seconds": t_acc,
         }
-        nrows = args.cpu_rows or max(cores * 512, 1024)
+        nrows = args.cpu_rows or max(cores * 3072, 4096)  # about 10 s of CPU work
         crow = np.arange(nrows, dtype=np.int64) * (n // nrows)
         t = time.perf_counter()
         oracle.ref_forces(pos, mass, G=1.0, S=soft, rows=crow, workers=cores)
