@@ -337,6 +337,7 @@ def bench_serve(pu, args):
     st = srv.stats()
     wall = srv.clock() - t0
     srv.close()
+    seq = serve_sequential(pu, args, sizes, steps, progress) if args.seq_every > 0 else None
     lat = np.array([r["completed"] - r["arrival"] for r in results])
     direct = [r for r in results if r["method"] == pu.METHOD_DIRECT]
     bh = [r for r in results if r["method"] == pu.METHOD_BARNES_HUT]
@@ -371,6 +372,8 @@ def bench_serve(pu, args):
             "p50_bh": pct([r["completed"] - r["arrival"] for r in bh], 50),
             "p99_bh": pct([r["completed"] - r["arrival"] for r in bh], 99),
         },
+        "sequential_baseline": seq,
+        "vs_sequential": (inter / wall) / seq["value"] if seq else None,
         "load": {
             "offered_requests": m,
             "submitted": int(st["submitted"]),
@@ -383,6 +386,47 @@ def bench_serve(pu, args):
             "memory_peak_gib": st["memory_peak"] / 2 ** 30,
             "timeline_s_pending_activeD_activeBH_mem": timeline[:: max(1, len(timeline) // 20)],
         },
+    }
+
+
+def serve_sequential(pu, args, sizes, steps, progress):
+    """Every `seq_every`-th request of the served list run alone, one after
+    another: one direct and one Barnes-Hut handle (reused, so no allocation per
+    request), a device-generated Plummer sphere, all steps enqueued, then a
+    synchronisation.  Interactions are counted as the server counts them (N^2
+    per direct step, counted cell and particle interactions for Barnes-Hut)."""
+    sel = np.arange(0, len(sizes), args.seq_every)
+    soft = args.eps ** 2
+    cfg_d = pu.default_config(law=pu.LAW_R, softening=soft)
+    cfg_b = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=soft, theta=0.6)
+    inter = 0.0
+    with pu.NBodySim(cfg_d) as sd, pu.NBodySim(cfg_b) as sb:
+        sb.set_timing(True)
+        t0 = time.perf_counter()
+        last = t0
+        for k, i in enumerate(sel):
+            n, ns = int(sizes[i]), int(steps[i])
+            sim = sd if n < 32768 else sb
+            sim.init_plummer(n, args.seed * 1000003 + int(i))
+            sim.step(args.dt, ns)
+            sim.synchronize()
+            inter += float(n) * n * ns if sim is sd else float(sim.timing().interactions)
+            now = time.perf_counter()
+            if now - last >= 5.0:
+                last = now
+                line = f"serve sequential {k + 1}/{len(sel)} requests, {now - t0:.1f}s"
+                print(line, file=sys.stderr, flush=True)
+                if progress:
+                    progress.write(line + "\n")
+                    progress.flush()
+        secs = time.perf_counter() - t0
+    return {
+        "value": inter / secs,
+        "unit": "interactions/s",
+        "requests": int(len(sel)),
+        "seconds": secs,
+        "kind": f"every {args.seq_every}th request of the same list alone and in turn (single-system handles, "
+        "plain launches, device-generated initial conditions, one synchronisation per request)",
     }
 
 
@@ -428,6 +472,8 @@ def main():
     ap.add_argument("--max-bh", type=int, default=16, help="serve: concurrent Barnes-Hut requests")
     ap.add_argument("--bh-ahead", type=int, default=1, help="serve: Barnes-Hut steps enqueued per request at a time")
     ap.add_argument("--progress", default="", help="append serve progress lines to this file")
+    ap.add_argument("--seq-every", type=int, default=40,
+                    help="serve: run every k-th request alone as the sequential baseline (0 = skip)")
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--subsample", type=int, default=65536)
     ap.add_argument("--galaxies", type=int, default=512)
