@@ -360,10 +360,10 @@ def test_cell_export_describes_the_octree(gpu):
     assert np.all(np.diff(count)[same] < 0)
 
 
-@pytest.mark.parametrize("n", [524288, 524289, 1048575, 1048576])
+@pytest.mark.parametrize("n", [32768, 32769, 1048575, 1048576])
 def test_sort_and_mass_paths_at_their_thresholds(gpu, oracle, n):
-    """The sort's scatter scans its own digit counts up to 256 tiles of 2048
-    keys (N <= 524,288) and uses a separate scan of the counts above; the mass prefix
+    """The sort's scatter scans its own digit counts up to 16 tiles of 2048
+    keys (N <= 32,768) and uses a separate scan of the counts above; the mass prefix
     folds its chunk scan up to 256 chunks of 4096 (N + 1 <= 1,048,576).  On
     both sides of both thresholds the keys and the stable order are bit-exact
     and the cells' FP64 masses are the sums of their particles' masses."""
