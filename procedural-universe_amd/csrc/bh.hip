@@ -69,8 +69,20 @@ namespace {
 constexpr int kLevels = 21;
 constexpr int kBucketLevel = kLevels;
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 8;
-constexpr int kSortTile = kSortThreads * kSortItems;
+// Keys per thread of a sort tile (tile = 256 x items keys): small systems take
+// small tiles for more workgroups per pass, large ones large tiles for fewer
+// tile counts to scan (measured tree builds: 0.19 / 0.21 / 0.24 ms at 65,536
+// particles with 2 / 4 / 8 items, 1.66 / 1.43 / 1.30 ms at 4,194,304).
+inline int sort_items(size_t n) { return n <= (size_t(1) << 17) ? 2 : n <= (size_t(1) << 20) ? 4 : 8; }
+// The most tiles of any system of at most n particles (the tile grows with
+// the size, so a smaller system can have more tiles than a larger one): the
+// count buffer of a handle holds that many, whatever size it runs later.
+inline size_t sort_max_tiles(size_t n) {
+    const auto tiles = [](size_t k, size_t tile) { return (k + tile - 1) / tile; };
+    size_t t = tiles(std::min(n, size_t(1) << 17), kSortThreads * 2);
+    t = std::max(t, tiles(std::min(n, size_t(1) << 20), kSortThreads * 4));
+    return std::max(t, tiles(n, kSortThreads * 8));
+}
 constexpr int kTravThreads = 64;  // one wavefront per workgroup: the finest tail
 constexpr int kTravWaves = kTravThreads / 64;
 // Pending (cell, lanes) entries per wave.  Popping one cell at a time (a
@@ -283,14 +295,15 @@ __global__ __launch_bounds__(256) void keys_kernel(const float4* __restrict__ po
 // ---- 3. radix sort -----------------------------------------------------------
 
 // Digit counts of each tile, digit-major (hist[d * nblocks + tile]).
+template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const unsigned long long* __restrict__ keys, int n,
                                                                   int shift, unsigned* __restrict__ hist,
                                                                   int nblocks) {
     __shared__ unsigned h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const int base = blockIdx.x * kSortTile;
-    for (int k = 0; k < kSortItems; ++k) {
+    const int base = blockIdx.x * kSortThreads * ITEMS;
+    for (int k = 0; k < ITEMS; ++k) {
         const int i = base + k * kSortThreads + threadIdx.x;
         if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
     }
@@ -366,8 +379,8 @@ __global__ void publish_counters_kernel(const unsigned* __restrict__ counters,
     if (threadIdx.x >= 3 && threadIdx.x < 5) host[threadIdx.x] = counters[threadIdx.x - 3];
 }
 
-// One radix pass over a tile of kSortTile keys, stable.  Each wavefront owns a
-// contiguous share of the tile (kSortItems rounds of 64 consecutive keys) and
+// One radix pass over a tile of 256 x ITEMS keys, stable.  Each wavefront owns
+// a contiguous share of the tile (ITEMS rounds of 64 consecutive keys) and
 // ranks every key among the earlier keys of its wavefront with the same digit
 // (the lanes with equal digits found by 8 ballots, running per-digit counts in
 // LDS; no workgroup barrier).  One pass over the 256 digits then gives each
@@ -384,20 +397,20 @@ __global__ void publish_counters_kernel(const unsigned* __restrict__ counters,
 // Either way one scan over the digits' totals gives the digits' starts.
 // GATHER (the last pass): the sorted positions are written beside the keys.
 constexpr int kSortWaves = kSortThreads / 64;
-constexpr int kSortPerWave = kSortTile / kSortWaves;  // = 64 * kSortItems
 
 // Above this many tiles the row scan's extra launch costs less than every
 // workgroup summing the digits' rows itself (measured: equal at 16 tiles,
 // 0.01 ms less at 32, 0.04 at 64, 0.24 at 256).
 constexpr int kFusedScanTiles = 16;
 
-template <bool FUSED, bool GATHER>
+template <bool FUSED, bool GATHER, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const unsigned long long* __restrict__ keys_in, const unsigned* __restrict__ idx_in,
     unsigned long long* __restrict__ keys_out, unsigned* __restrict__ idx_out, int n, int shift,
     const unsigned* __restrict__ counts, const unsigned* __restrict__ totals, int nblocks,
     const float4* __restrict__ pos_in, float4* __restrict__ pos_out) {
-    static_assert(kSortPerWave == 64 * kSortItems, "a wavefront's share is kSortItems rounds of 64 keys");
+    constexpr int kSortTile = kSortThreads * ITEMS;
+    constexpr int kSortPerWave = 64 * ITEMS;  // a wavefront's share: ITEMS rounds of 64 keys
     __shared__ unsigned cnt[kSortWaves][256];  // per-wave digit counts, then per-wave offsets in the run
     __shared__ unsigned run_start[256];        // the tile's digit runs
     __shared__ unsigned dst_base[256];         // the runs' places in the output
@@ -410,10 +423,10 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const int wbase = base + wave * kSortPerWave;
 #pragma unroll
     for (int d = lane; d < 256; d += 64) cnt[wave][d] = 0u;  // this wave's counters only
-    unsigned long long key[kSortItems];
-    unsigned idx[kSortItems], rank[kSortItems];
+    unsigned long long key[ITEMS];
+    unsigned idx[ITEMS], rank[ITEMS];
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
+    for (int r = 0; r < ITEMS; ++r) {
         const int i = wbase + r * 64 + lane;
         const bool valid = i < n;
         key[r] = valid ? keys_in[i] : 0ULL;
@@ -484,7 +497,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
+    for (int r = 0; r < ITEMS; ++r) {
         const int i = wbase + r * 64 + lane;
         if (i < n) {
             const unsigned digit = static_cast<unsigned>((key[r] >> shift) & 255u);
@@ -496,7 +509,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     __syncthreads();
     const int count = min(kSortTile, n - base);
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
+    for (int r = 0; r < ITEMS; ++r) {
         const int at = r * kSortThreads + tid;
         if (at < count) {
             const unsigned long long k = skeys[at];
@@ -1545,7 +1558,7 @@ nbody_status bh_resize(BHState* b, size_t n, nbody_sim* s) {
     NB_TRY(bh_alloc(b->keys[1], n));
     NB_TRY(bh_alloc(b->idx[0], n));
     NB_TRY(bh_alloc(b->idx[1], n));
-    const size_t nb = static_cast<size_t>(blocks_for(n, kSortTile));
+    const size_t nb = sort_max_tiles(n);
     NB_TRY(bh_alloc(b->hist, 256 * nb));
     b->hist_elems = 256 * nb;
     NB_TRY(bh_alloc(b->pos_sorted, n));
@@ -1569,6 +1582,31 @@ nbody_status bh_resize(BHState* b, size_t n, nbody_sim* s) {
     return NBODY_OK;
 }
 
+// The eight LSD passes over the keys in tiles of 256 x ITEMS (the last pass
+// also gathers the sorted positions); the sorted buffer is b->sorted_buf.
+template <int ITEMS>
+void radix_sort_passes(BHState* b, int n, const float4* pos, hipStream_t st) {
+    const int nb = blocks_for(n, kSortThreads * ITEMS);
+    int cur = 0;
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 8 * pass;
+        // Above kFusedScanTiles tiles the tile offsets within each digit and
+        // the digits' totals come from a separate scan of the counts.
+        const bool fused = nb <= kFusedScanTiles;
+        hipLaunchKernelGGL(radix_hist_kernel<ITEMS>, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], n, shift, b->hist, nb);
+        if (!fused)
+            hipLaunchKernelGGL(radix_rowscan_kernel, dim3(256), dim3(kScanThreads2), 0, st, b->hist, nb,
+                               b->digit_totals);
+        // the last pass also writes the sorted positions
+        auto scatter = pass < 7 ? (fused ? radix_scatter_kernel<true, false, ITEMS> : radix_scatter_kernel<false, false, ITEMS>)
+                                : (fused ? radix_scatter_kernel<true, true, ITEMS> : radix_scatter_kernel<false, true, ITEMS>);
+        hipLaunchKernelGGL(scatter, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], b->idx[cur], b->keys[cur ^ 1],
+                           b->idx[cur ^ 1], n, shift, b->hist, b->digit_totals, nb, pos, b->pos_sorted);
+        cur ^= 1;
+    }
+    b->sorted_buf = cur;
+}
+
 // Keys, sort, tree, masses: everything before the traversal.
 static nbody_status bh_build(nbody_sim* s, BHState* b) {
     const int n = static_cast<int>(s->n);
@@ -1582,27 +1620,13 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     }
     hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, pos, n, b->root, b->keys[0],
                        b->idx[0], b->counters);
-    const int nb = blocks_for(n, kSortTile);
-    int cur = 0;
-    for (int pass = 0; pass < 8; ++pass) {
-        const int shift = 8 * pass;
-        // Above kFusedScanTiles tiles the tile offsets within each digit and
-        // the digits' totals come from a separate scan of the counts.
-        const bool fused = nb <= kFusedScanTiles;
-        hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], n, shift, b->hist, nb);
-        if (!fused)
-            hipLaunchKernelGGL(radix_rowscan_kernel, dim3(256), dim3(kScanThreads2), 0, st, b->hist, nb,
-                               b->digit_totals);
-        // the last pass also writes the sorted positions
-        auto scatter = pass < 7 ? (fused ? radix_scatter_kernel<true, false> : radix_scatter_kernel<false, false>)
-                                : (fused ? radix_scatter_kernel<true, true> : radix_scatter_kernel<false, true>);
-        hipLaunchKernelGGL(scatter, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], b->idx[cur], b->keys[cur ^ 1],
-                           b->idx[cur ^ 1], n, shift, b->hist, b->digit_totals, nb, pos, b->pos_sorted);
-        cur ^= 1;
+    switch (sort_items(static_cast<size_t>(n))) {
+    case 2: radix_sort_passes<2>(b, n, pos, st); break;
+    case 4: radix_sort_passes<4>(b, n, pos, st); break;
+    default: radix_sort_passes<8>(b, n, pos, st); break;
     }
-    b->sorted_buf = cur;
-    const unsigned long long* keys = b->keys[cur];
-    const unsigned* order = b->idx[cur];
+    const unsigned long long* keys = b->keys[b->sorted_buf];
+    const unsigned* order = b->idx[b->sorted_buf];
     const int m = n - 1;
     TreeArgs T{keys, n, b->child, b->first, b->last, b->info};
     const int tree_blocks = blocks_for(m, 256);

@@ -360,13 +360,15 @@ def test_cell_export_describes_the_octree(gpu):
     assert np.all(np.diff(count)[same] < 0)
 
 
-@pytest.mark.parametrize("n", [32768, 32769, 1048575, 1048576])
+@pytest.mark.parametrize("n", [8192, 8193, 131072, 131073, 1048575, 1048576, 1048577])
 def test_sort_and_mass_paths_at_their_thresholds(gpu, oracle, n):
-    """The sort's scatter scans its own digit counts up to 16 tiles of 2048
-    keys (N <= 32,768) and uses a separate scan of the counts above; the mass prefix
-    folds its chunk scan up to 256 chunks of 4096 (N + 1 <= 1,048,576).  On
-    both sides of both thresholds the keys and the stable order are bit-exact
-    and the cells' FP64 masses are the sums of their particles' masses."""
+    """The sort's tiles hold 512 keys up to N = 131,072, 1,024 up to
+    1,048,576 and 2,048 above; its scatter scans its own digit counts up to 16
+    tiles (N <= 8,192) and uses a separate scan of the counts above; the mass
+    prefix folds its chunk scan up to 256 chunks of 4096 (N + 1 <= 1,048,576).
+    On both sides of each threshold the keys and the stable order are
+    bit-exact and the cells' FP64 masses are the sums of their particles'
+    masses."""
     pu = gpu
     pos, vel, mass = pu.ic_plummer(n, 0x5EED0300 + (n & 0xff))
     cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=1e-4, theta=0.5)
