@@ -1,4 +1,5 @@
-"""The Barnes-Hut walk's stack never overflows, checked on the CPU.
+"""The Barnes-Hut walk's stack and interaction list never overflow, checked on
+the CPU.
 
 `traverse_kernel` (csrc/bh.hip) pops a batch of up to eight cells from the top of
 a wave's LDS stack of `kStackCap` entries and pushes, in their place, the opened
@@ -27,6 +28,11 @@ the arithmetic, and runs the same pop/push discipline over adversarial trees
 (full octrees that fill the stack in batches, topped by deep combs that then
 descend depth-first), where the bound must hold and be approached; the same
 trees overflow a reserve sized for fourteen levels, so the simulation can fail.
+
+The interaction list (`kListCap` entries per wave) takes at most eight entries
+per cell of a batch and one per particle of a bucket; its guards (flush or a
+smaller batch) are read from the kernel source and checked for every list
+length and batch size, together with the flush's padding to a multiple of four.
 """
 import pathlib
 import re
@@ -54,6 +60,17 @@ def kernel_constants():
     levels = _int_expr(re.search(r"constexpr int kLevels = ([^;]+);", src).group(1))
     rule = re.search(r"const int kmax = ([^;]+);", src).group(1)
     return cap, reserve, levels, rule
+
+
+def list_guards():
+    """kListCap and the two guards in front of list appends."""
+    src = SRC.read_text()
+    cap = _int_expr(re.search(r"constexpr int kListCap = ([^;]+);", src).group(1))
+    bucket = re.search(r"if \(cnt > kListCap - (\d+)\) flush\(\);", src)
+    batch = re.search(r"if \(cnt > kListCap - 8 \* k\) \{\s*const int fit = \(kListCap - cnt\) >> 3;\s*"
+                      r"if \(fit >= (\d+)\) k = fit;\s*else flush\(\);", src)
+    assert bucket and batch, "list guards not found in the kernel source"
+    return cap, int(bucket.group(1)), int(batch.group(1))
 
 
 def proof_bound(cap, reserve, levels):
@@ -175,3 +192,39 @@ def test_the_same_trees_overflow_a_reserve_for_fewer_levels():
     cap, _, levels, _ = kernel_constants()
     results = [walk(root, cap, 7 * 14, levels) for root in adversarial_roots(levels)]
     assert any(over for _, over in results)
+
+
+def padded(cnt):
+    return cnt + ((4 - (cnt & 3)) & 3)
+
+
+def test_list_guards_keep_every_append_inside_the_list():
+    cap, bucket_margin, min_fit = list_guards()
+    assert cap % 4 == 0
+    reachable = set()
+    for cnt in range(cap + 1):
+        # a bucket's particles: one entry each, after the guard
+        c = 0 if cnt > cap - bucket_margin else cnt
+        assert c + 1 <= cap
+        reachable.add(c + 1)
+        # a batch of k cells: at most eight entries each, after the guard
+        for k in range(1, 9):
+            c, kk = cnt, k
+            if c > cap - 8 * kk:
+                fit = (cap - c) >> 3
+                if fit >= min_fit:
+                    kk = fit
+                else:
+                    c = 0
+            assert 1 <= kk <= k
+            assert c + 8 * kk <= cap, (cnt, k)
+            reachable.update(range(c, c + 8 * kk + 1))
+    # the flush pads with zero entries up to a multiple of four
+    assert all(padded(c) <= cap for c in reachable)
+
+
+def test_list_guards_without_the_bucket_margin_would_overflow():
+    """The same exhaustive check catches a bucket guard that flushes only when
+    the list is already full."""
+    cap, _, _ = list_guards()
+    assert any((0 if cnt > cap else cnt) + 1 > cap for cnt in range(cap + 1))
