@@ -244,6 +244,14 @@ typedef struct nbody_timing {
  * counterpart in the reference; for callers that step many small handles. */
 nbody_status nbody_set_graph(nbody_sim* sim, int32_t enable);
 
+/* Octree::Theta of a live simulation, the reference's BHThetaChanged handler
+ * (src/Sim/BarnesHut.cpp:29-31 sets the global Octree::Theta, Octree.cpp:5,
+ * from the UI's slider, UI.cpp:105-108,193).  Waits for the steps already
+ * enqueued on the handle (they keep the old theta), then every later step
+ * opens cells with the new one; captured step graphs are dropped.  theta must
+ * be finite and >= 0.  A direct-sum handle keeps the value and ignores it. */
+nbody_status nbody_set_theta(nbody_sim* sim, double theta);
+
 /* Turn per-kernel HIP-event timing of nbody_step on or off (default off). */
 nbody_status nbody_set_timing(nbody_sim* sim, int32_t enable);
 

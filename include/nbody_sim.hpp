@@ -83,6 +83,13 @@ public:
 
     nbody_sim* handle() { return sim_; }
 
+    // The reference's BHThetaChanged handler (BarnesHut.cpp:29-31) sets the
+    // global Octree::Theta from the UI's slider; here the handle's theta.
+    // Register it where the reference registers its handler:
+    //     EventStream::Register(EEvent::BHThetaChanged, [&](const EventData& d) {
+    //         sim.SetTheta(EventValue<FloatEventData>(d)); });
+    void SetTheta(double theta) { check("nbody_set_theta", nbody_set_theta(sim_, theta)); }
+
     // The octree cells of the last Barnes-Hut step (cube corner and side,
     // centre of mass, mass, level, particle range): what the reference's
     // Octree::RenderDebug draws (Octree.cpp:147-175).  Empty for the direct sum.

@@ -270,6 +270,7 @@ _SIGNATURES = {
     "nbody_synchronize": (ctypes.c_int, [_P]),
     "nbody_query": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32)]),
     "nbody_set_graph": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "nbody_set_theta": (ctypes.c_int, [_P, ctypes.c_double]),
     "nbody_init_seeded": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_double]),
     "nbody_read": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.c_uint32]),
     "nbody_read_soa": (ctypes.c_int, [_P, _F32P, _F32P, _F32P, ctypes.c_size_t]),
@@ -518,6 +519,11 @@ class NBodySim:
     def set_graph(self, enable: bool = True):
         """Replay step sequences from captured hipGraphs (bit-identical results)."""
         _check("nbody_set_graph", self._lib.nbody_set_graph(self._h, 1 if enable else 0))
+
+    def set_theta(self, theta: float):
+        """Octree::Theta of the live simulation (the reference's BHThetaChanged,
+        src/Sim/BarnesHut.cpp:29-31): later steps open cells with it."""
+        _check("nbody_set_theta", self._lib.nbody_set_theta(self._h, float(theta)))
 
     def read_soa(self):
         n = self.n

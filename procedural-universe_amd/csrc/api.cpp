@@ -94,7 +94,13 @@ nbody_status dev_alloc(T** p, size_t count) {
     return NBODY_OK;
 }
 
+// A graph exec is only destroyed once its launches have completed: every
+// launch went to the handle's stream, so waiting for the stream suffices.
 void drop_graphs(nbody_sim* s) {
+    bool any = false;
+    for (const auto& g : s->graphs) any = any || g.exec;
+    if (!any) return;
+    (void)hipStreamSynchronize(s->stream);
     for (auto& g : s->graphs) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         g = nbody_sim::StepGraph{};
@@ -160,6 +166,37 @@ nbody_status prepare_state(nbody_sim* s, size_t n) {
     return NBODY_OK;
 }
 
+// The largest coordinate range of n positions (x, y, z in .x .y .z).
+double extent_of(const float4* p, size_t n) {
+    if (n == 0) return 0.0;
+    float lo[3] = {p[0].x, p[0].y, p[0].z}, hi[3] = {p[0].x, p[0].y, p[0].z};
+    for (size_t i = 1; i < n; ++i) {
+        const float c[3] = {p[i].x, p[i].y, p[i].z};
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], c[a]);
+            hi[a] = std::max(hi[a], c[a]);
+        }
+    }
+    double e = 0.0;
+    for (int a = 0; a < 3; ++a) e = std::max(e, static_cast<double>(hi[a]) - static_cast<double>(lo[a]));
+    return e;
+}
+
+// The direct sum's scaled units for the handle's state (extent and max_gm
+// of the initial state; a system that later grows or shrinks by several
+// orders of magnitude keeps them, which the pair arithmetic's range allows).
+void set_scale(nbody_sim* s, double extent) {
+    s->dscale = direct_scale_for(s->cfg.law, s->cfg.softening, extent, s->max_gm);
+}
+
+// Masses must be finite and >= 0 (the direct sum folds G m into a square).
+nbody_status check_masses(const double* mass, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        NB_CHECK(std::isfinite(mass[i]) && mass[i] >= 0, NBODY_ERR_INVALID_ARG, "mass[%zu] = %g must be finite and >= 0",
+                 i, mass[i]);
+    return NBODY_OK;
+}
+
 // Upload a packed state (n real records) into freshly sized device buffers.
 nbody_status upload_state(nbody_sim* s, const std::vector<float4>& pos, const std::vector<float4>& vel,
                           const std::vector<double>& mass, size_t n) {
@@ -175,6 +212,7 @@ nbody_status upload_state(nbody_sim* s, const std::vector<float4>& pos, const st
         hg[i] = s->cfg.G * mass[i];
         s->max_gm = std::max(s->max_gm, std::fabs(hg[i]));
     }
+    set_scale(s, extent_of(pos.data(), n));
     NB_HIP(hipMemcpyAsync(s->pos[0], hp.data(), n_pad * sizeof(float4), hipMemcpyHostToDevice, s->stream));
     NB_HIP(hipMemcpyAsync(s->pos[1], hp.data(), n_pad * sizeof(float4), hipMemcpyHostToDevice, s->stream));
     NB_HIP(hipMemcpyAsync(s->vel, hv.data(), n_pad * sizeof(float4), hipMemcpyHostToDevice, s->stream));
@@ -208,8 +246,7 @@ DirectLaunch launch_for(nbody_sim* s, float dt) {
     L.n = static_cast<int>(s->n);
     L.n_pad = static_cast<int>(s->n_pad);
     L.law = s->cfg.law;
-    L.soft = static_cast<float>(s->cfg.softening);
-    L.tiny = direct_tiny_for(s->max_gm);
+    L.sc = s->dscale;
     L.dt = dt;
     L.dts = static_cast<float>(static_cast<double>(dt) / s->cfg.scale);
     return L;
@@ -318,6 +355,8 @@ nbody_status nbody_init(nbody_sim* s, const nbody_particle* p, size_t n) {
     NB_CHECK(n <= (size_t(1) << 30), NBODY_ERR_CAPACITY, "n = %zu exceeds 2^30", n);
     std::vector<float4> pos(n), vel(n);
     std::vector<double> mass(n);
+    for (size_t i = 0; i < n; ++i) mass[i] = p[i].mass;
+    NB_TRY(check_masses(mass.data(), n));
     for (size_t i = 0; i < n; ++i) {
         const double gm = s->cfg.G * p[i].mass;
         pos[i] = make_float4(p[i].position[0], p[i].position[1], p[i].position[2], static_cast<float>(gm));
@@ -333,6 +372,7 @@ nbody_status nbody_init_soa(nbody_sim* s, const float* pos3, const float* vel3, 
     NB_CHECK(s, NBODY_ERR_INVALID_ARG, "sim is null");
     NB_CHECK((pos3 && mass) || n == 0, NBODY_ERR_INVALID_ARG, "pos or mass is null");
     NB_CHECK(n <= (size_t(1) << 30), NBODY_ERR_CAPACITY, "n = %zu exceeds 2^30", n);
+    NB_TRY(check_masses(mass, n));
     std::vector<float4> pos(n), vel(n);
     std::vector<double> m(n);
     for (size_t i = 0; i < n; ++i) {
@@ -364,6 +404,7 @@ nbody_status nbody_init_plummer(nbody_sim* s, uint64_t seed, size_t n, double a,
     const double m = total_mass / static_cast<double>(n);
     s->mass_host.assign(n, m);
     s->max_gm = std::fabs(s->cfg.G * m);
+    set_scale(s, 40.0 * a);  // radii are cut at 20 a
     s->initialised = true;
     s->acc_valid = false;
     if (s->bh) NB_TRY(bh_resize(s->bh, n, s));
@@ -378,7 +419,11 @@ nbody_status step_graph(nbody_sim* s, float dt, int32_t nsteps) {
     auto& G = s->graphs[s->cur];
     const int parity = s->cur;
     if (!(G.exec && G.steps == nsteps && G.dt == dt)) {
-        if (G.exec) (void)hipGraphExecDestroy(G.exec);
+        if (G.exec) {
+            // its last launch may still be queued on the stream
+            NB_HIP(hipStreamSynchronize(s->stream));
+            (void)hipGraphExecDestroy(G.exec);
+        }
         G = nbody_sim::StepGraph{};
         hipGraph_t g = nullptr;
         NB_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
@@ -431,10 +476,13 @@ nbody_status nbody_init_seeded(nbody_sim* s, int32_t seeder, uint64_t seed, size
     NB_HIP(ic_device_seeded(seeder, seed, static_cast<int>(n), static_cast<float>(scale), s->cfg.G, T, s->pos[0],
                             s->vel, s->mass_dev, s->gm_dev, s->pos[1], s->stream));
     s->mass_host.resize(n);
+    std::vector<float4> pos_host(n);
     NB_HIP(hipMemcpyAsync(s->mass_host.data(), s->mass_dev, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    NB_HIP(hipMemcpyAsync(pos_host.data(), s->pos[0], n * sizeof(float4), hipMemcpyDeviceToHost, s->stream));
     NB_HIP(hipStreamSynchronize(s->stream));
     s->max_gm = 0.0;
     for (double m : s->mass_host) s->max_gm = std::max(s->max_gm, std::fabs(s->cfg.G * m));
+    set_scale(s, extent_of(pos_host.data(), n));
     s->initialised = true;
     s->acc_valid = false;
     if (s->bh) NB_TRY(bh_resize(s->bh, n, s));
@@ -572,8 +620,8 @@ nbody_status nbody_field(nbody_sim* s, const float* points, size_t m, float* acc
     nbody_status st = NBODY_OK;
     hipError_t e = hipMemcpyAsync(d, h.data(), m * sizeof(float4), hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess)
-        e = direct_field(s->pos[s->cur], static_cast<int>(s->n_pad), d, static_cast<int>(m), s->cfg.law,
-                         static_cast<float>(s->cfg.softening), direct_tiny_for(s->max_gm), d + m, s->stream);
+        e = direct_field(s->pos[s->cur], static_cast<int>(s->n_pad), d, static_cast<int>(m), s->cfg.law, s->dscale,
+                         d + m, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d + m, m * sizeof(float4), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) {
@@ -645,6 +693,20 @@ nbody_status nbody_set_graph(nbody_sim* s, int32_t enable) {
     NB_HIP(hipStreamSynchronize(s->stream));
     s->use_graph = enable != 0;
     if (!s->use_graph) drop_graphs(s);
+    return NBODY_OK;
+}
+
+nbody_status nbody_set_theta(nbody_sim* s, double theta) {
+    clear_error();
+    NB_CHECK(s, NBODY_ERR_INVALID_ARG, "sim is null");
+    NB_CHECK(std::isfinite(theta) && theta >= 0, NBODY_ERR_INVALID_ARG, "theta must be finite and >= 0");
+    NB_HIP(hipSetDevice(s->cfg.device));
+    NB_HIP(hipStreamSynchronize(s->stream));
+    // A captured Barnes-Hut step holds theta (the dynamic root's kernel
+    // argument), so the graphs are captured again on the next step.
+    drop_graphs(s);
+    s->cfg.theta = theta;
+    if (s->bh) NB_TRY(bh_set_theta(s));
     return NBODY_OK;
 }
 

@@ -34,10 +34,11 @@ namespace nbody {
 namespace {
 
 struct SysDesc {
-    int off;     // first record in the arena
-    int n;       // real particles
-    int n_pad;   // padded to kTile
-    float tiny;  // law-R guard for this system
+    int off;         // first record in the arena
+    int n;           // real particles
+    int n_pad;       // padded to kTile
+    int pad_;
+    DirectScale sc;  // the system's scaled units
 };
 
 struct WorkItem {
@@ -53,7 +54,6 @@ struct BatchLaunch {
     const SysDesc* sys;
     const WorkItem* items;
     const int* n_items;
-    float soft;
     float dt;
     float dts;
 };
@@ -63,13 +63,14 @@ __global__ void fill_u32_kernel(unsigned* p, int n, unsigned v) {
     if (i < n) p[i] = v;
 }
 
+// The barriers around the item fetch also separate one item's last reads of
+// the LDS tiles from the next item's first writes.
 template <int LAW, int K, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void batch_step_kernel(BatchLaunch B, unsigned* counter) {
-    __shared__ float4 tile[kTile];
+    __shared__ core::SrcTile tiles[2];
     __shared__ int item_sh;
     const int tid = threadIdx.x;
     const int n_items = *B.n_items;
-    const float soft = core::in_vgpr(B.soft);
     for (;;) {
         if (tid == 0) item_sh = static_cast<int>(atomicAdd(counter, 1u));
         __syncthreads();
@@ -79,34 +80,34 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(BatchLaunch B, unsign
         const WorkItem w = B.items[it];
         const SysDesc d = B.sys[w.sys];
         const float4* src = B.pos_in + d.off;
-        const float tiny = core::in_vgpr(d.tiny);
         float xi[K], yi[K], zi[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int i = min(w.ibase + tid + k * BLOCK, d.n_pad - 1);
             const float4 q = src[i];
-            xi[k] = q.x;
-            yi[k] = q.y;
-            zi[k] = q.z;
+            xi[k] = q.x * d.sc.inv_len;
+            yi[k] = q.y * d.sc.inv_len;
+            zi[k] = q.z * d.sc.inv_len;
         }
         float sx[K], sy[K], sz[K];
-        core::block_forces<LAW, K, BLOCK>(tile, src, 0, d.n_pad, xi, yi, zi, soft, tiny, sx, sy, sz);
+        core::block_forces<LAW, K, BLOCK>(tiles, src, 0, d.n_pad, d.sc, xi, yi, zi, sx, sy, sz);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int i = w.ibase + tid + k * BLOCK;
             if (i < d.n) {
                 const int r = d.off + i;
+                const float ax = sx[k] * d.sc.out, ay = sy[k] * d.sc.out, az = sz[k] * d.sc.out;
                 float4 v = B.vel[r];
                 float4 x = src[i];
-                v.x = __builtin_fmaf(sx[k], B.dt, v.x);
-                v.y = __builtin_fmaf(sy[k], B.dt, v.y);
-                v.z = __builtin_fmaf(sz[k], B.dt, v.z);
+                v.x = __builtin_fmaf(ax, B.dt, v.x);
+                v.y = __builtin_fmaf(ay, B.dt, v.y);
+                v.z = __builtin_fmaf(az, B.dt, v.z);
                 x.x = __builtin_fmaf(v.x, B.dts, x.x);
                 x.y = __builtin_fmaf(v.y, B.dts, x.y);
                 x.z = __builtin_fmaf(v.z, B.dts, x.z);
                 B.vel[r] = v;
                 B.pos_out[r] = x;
-                B.acc[r] = make_float4(sx[k], sy[k], sz[k], 0.f);
+                B.acc[r] = make_float4(ax, ay, az, 0.f);
             }
         }
     }
@@ -162,7 +163,7 @@ struct nbody_batch {
     struct Sys {
         bool live = false;
         size_t off = 0, n = 0, n_pad = 0;
-        float tiny = 0.f;
+        DirectScale sc{};
         double max_gm = 0.0;
     };
     std::vector<Sys> systems;  // index = id = descriptor slot
@@ -207,13 +208,23 @@ struct nbody_batch {
 namespace nbody {
 namespace {
 
-nbody_status ensure_dev(void** p, size_t& cap, size_t need, size_t elem) {
+// Grow a device buffer; the old one is freed once the steps queued on the
+// batch's stream (which may read it) have completed.
+nbody_status ensure_dev(void** p, size_t& cap, size_t need, size_t elem, hipStream_t stream) {
     if (need <= cap && *p) return NBODY_OK;
     size_t ncap = std::max<size_t>(need, cap * 2);
     if (ncap == 0) ncap = 1;
     void* q = nullptr;
     NB_HIP(hipMalloc(&q, ncap * elem));
-    if (*p) (void)hipFree(*p);
+    if (*p) {
+        const hipError_t e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) {
+            (void)hipFree(q);
+            set_error("hipStreamSynchronize failed: %s", hipGetErrorString(e));
+            return NBODY_ERR_DEVICE;
+        }
+        (void)hipFree(*p);
+    }
     *p = q;
     cap = ncap;
     return NBODY_OK;
@@ -265,7 +276,7 @@ nbody_status rebuild_items(nbody_batch* b) {
     uint64_t inter = 0;
     for (size_t s = 0; s < b->systems.size(); ++s) {
         const auto& y = b->systems[s];
-        descs[s] = SysDesc{static_cast<int>(y.off), static_cast<int>(y.n), static_cast<int>(y.n_pad), y.tiny};
+        descs[s] = SysDesc{static_cast<int>(y.off), static_cast<int>(y.n), static_cast<int>(y.n_pad), 0, y.sc};
     }
     for (int s : order) {
         const auto& y = b->systems[s];
@@ -273,9 +284,9 @@ nbody_status rebuild_items(nbody_batch* b) {
         for (size_t i = 0; i < y.n_pad; i += per) items.push_back(WorkItem{s, static_cast<int>(i)});
     }
     NB_TRY(ensure_dev(reinterpret_cast<void**>(&b->d_items), b->items_cap, std::max<size_t>(items.size(), 1),
-                      sizeof(WorkItem)));
+                      sizeof(WorkItem), b->stream));
     NB_TRY(ensure_dev(reinterpret_cast<void**>(&b->d_sys), b->sys_cap, std::max<size_t>(descs.size(), 1),
-                      sizeof(SysDesc)));
+                      sizeof(SysDesc), b->stream));
     // The graph holds the item and descriptor pointers: a reallocation above
     // invalidates it.
     if (!items.empty())
@@ -292,7 +303,13 @@ nbody_status rebuild_items(nbody_batch* b) {
     return NBODY_OK;
 }
 
+// A graph exec is only destroyed once its launches have completed (they all
+// went to the batch's stream).
 void drop_graph(nbody_batch* b) {
+    bool any = false;
+    for (const auto& g : b->graph) any = any || g.exec;
+    if (!any) return;
+    (void)hipStreamSynchronize(b->stream);
     for (auto& g : b->graph) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         g = nbody_batch::Graph{};
@@ -308,7 +325,6 @@ BatchLaunch launch_for(nbody_batch* b, int parity, float dt) {
     L.sys = b->d_sys;
     L.items = b->d_items;
     L.n_items = b->d_n_items;
-    L.soft = static_cast<float>(b->cfg.softening);
     L.dt = dt;
     L.dts = static_cast<float>(static_cast<double>(dt) / b->cfg.scale);
     return L;
@@ -397,18 +413,27 @@ nbody_status nbody_batch_add(nbody_batch* b, const float* pos3, const float* vel
     clear_error();
     NB_CHECK(b && id, NBODY_ERR_INVALID_ARG, "null argument");
     NB_CHECK(n > 0 && pos3 && mass, NBODY_ERR_INVALID_ARG, "a system needs n > 0 particles with positions and masses");
+    for (size_t i = 0; i < n; ++i)
+        NB_CHECK(mass[i] >= 0 && std::isfinite(mass[i]), NBODY_ERR_INVALID_ARG, "mass[%zu] must be finite and >= 0", i);
     NB_HIP(hipSetDevice(b->cfg.device));
     const size_t n_pad = round_up(n, kTile);
     size_t off = 0;
     NB_TRY(arena_alloc(b, n_pad, &off));
     std::vector<float4> hp(n_pad, make_float4(0.f, 0.f, 0.f, 0.f)), hv(n_pad, make_float4(0.f, 0.f, 0.f, 0.f));
     double max_gm = 0.0;
+    float lo[3] = {pos3[0], pos3[1], pos3[2]}, hi[3] = {pos3[0], pos3[1], pos3[2]};
     for (size_t i = 0; i < n; ++i) {
         const double gm = b->cfg.G * mass[i];
         max_gm = std::max(max_gm, std::fabs(gm));
         hp[i] = make_float4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], static_cast<float>(gm));
         if (vel3) hv[i] = make_float4(vel3[3 * i], vel3[3 * i + 1], vel3[3 * i + 2], 0.f);
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(lo[c], pos3[3 * i + c]);
+            hi[c] = std::max(hi[c], pos3[3 * i + c]);
+        }
     }
+    const double extent = std::max({static_cast<double>(hi[0]) - lo[0], static_cast<double>(hi[1]) - lo[1],
+                                    static_cast<double>(hi[2]) - lo[2]});
     // Enqueued behind any running step: the segment was free, nothing reads it.
     hipError_t e = hipSuccess;
     for (int k = 0; k < 2 && e == hipSuccess; ++k)
@@ -434,7 +459,7 @@ nbody_status nbody_batch_add(nbody_batch* b, const float* pos3, const float* vel
     y.n = n;
     y.n_pad = n_pad;
     y.max_gm = max_gm;
-    y.tiny = direct_tiny_for(max_gm);
+    y.sc = direct_scale_for(b->cfg.law, b->cfg.softening, extent, max_gm);
     b->items_dirty = true;
     *id = slot;
     return NBODY_OK;
@@ -477,7 +502,8 @@ nbody_status nbody_batch_add_plummer(nbody_batch* b, uint64_t seed, size_t n, do
     y.n = n;
     y.n_pad = n_pad;
     y.max_gm = std::fabs(b->cfg.G * total_mass / static_cast<double>(n));
-    y.tiny = direct_tiny_for(y.max_gm);
+    // Plummer radii are cut at 20 a, so the sphere spans at most 40 a
+    y.sc = direct_scale_for(b->cfg.law, b->cfg.softening, 40.0 * a, y.max_gm);
     b->items_dirty = true;
     *id = slot;
     return NBODY_OK;
@@ -515,6 +541,7 @@ nbody_status nbody_batch_step(nbody_batch* b, float dt, int32_t nsteps) {
     if (b->n_items == 0) return NBODY_OK;
     if (static_cast<int>(nsteps) > b->counters_cap) {
         drop_graph(b);
+        NB_HIP(hipStreamSynchronize(b->stream));  // plain launches may still use the counters
         if (b->d_counters) (void)hipFree(b->d_counters);
         b->d_counters = nullptr;
         NB_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_counters), static_cast<size_t>(nsteps) * sizeof(unsigned)));
@@ -526,7 +553,11 @@ nbody_status nbody_batch_step(nbody_batch* b, float dt, int32_t nsteps) {
         const bool reuse = G.exec && G.steps == nsteps && G.dt == dt && G.k == b->tuning.i_per_lane &&
                            G.block == b->tuning.block && G.wg == b->tuning.wg_per_cu;
         if (!reuse) {
-            if (G.exec) (void)hipGraphExecDestroy(G.exec);
+            if (G.exec) {
+                // its last launch may still be queued on the stream
+                NB_HIP(hipStreamSynchronize(b->stream));
+                (void)hipGraphExecDestroy(G.exec);
+            }
             G = nbody_batch::Graph{};
             hipGraph_t g = nullptr;
             NB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -596,8 +627,7 @@ nbody_status nbody_batch_step_sequential(nbody_batch* b, float dt, int32_t nstep
             L.n = static_cast<int>(y.n);
             L.n_pad = static_cast<int>(y.n_pad);
             L.law = b->cfg.law;
-            L.soft = static_cast<float>(b->cfg.softening);
-            L.tiny = y.tiny;
+            L.sc = y.sc;
             L.dt = dt;
             L.dts = static_cast<float>(static_cast<double>(dt) / b->cfg.scale);
             NB_HIP(direct_step(L, shape, true, b->stream, nullptr, nullptr));

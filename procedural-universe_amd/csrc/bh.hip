@@ -1046,19 +1046,17 @@ __device__ __forceinline__ float lane_gm(float gm, unsigned w, int lane) {
     return __int_as_float(__float_as_int(gm) & __builtin_amdgcn_sbfe(static_cast<int>(w), lane & 31, 1));
 }
 
+// v_writelane_b32 through the compiler's own intrinsic (this clang has no
+// __builtin_amdgcn_writelane): the compiler places the lane select in M0,
+// which it reserves and sets itself, and inserts the wait states its hazard
+// model requires between the ballot that produces the value and the write.
+__device__ int nb_writelane(int value, int lane, int old) __asm("llvm.amdgcn.writelane");
+
 // (lo, hi) = x in lane l only (register writes; the other lanes keep lo and
-// hi).  The lane number goes through M0: one instruction may read only one
-// SGPR besides M0.  M0 is reserved to the compiler, which sets it right before
-// each instruction that reads it; no instruction of the traversal kernels
-// does, so the asm may overwrite it.  The compiler does not look inside the
-// asm, so the asm waits for the hazards its operands can carry: x is usually a
-// compare result (a VALU write of VCC read by a VALU instruction as a scalar
-// operand needs one wait state, of another SGPR pair two), and a lane select
-// written just before needs up to four.
+// hi).  l and x are uniform over the wave.
 __device__ __forceinline__ void write_lane64(unsigned& lo, unsigned& hi, unsigned long long x, int l) {
-    asm("s_mov_b32 m0, %4\n\ts_nop 4\n\tv_writelane_b32 %0, %2, m0\n\tv_writelane_b32 %1, %3, m0"
-        : "+v"(lo), "+v"(hi)
-        : "s"(static_cast<unsigned>(x)), "s"(static_cast<unsigned>(x >> 32)), "s"(l));
+    lo = static_cast<unsigned>(nb_writelane(static_cast<int>(x), l, static_cast<int>(lo)));
+    hi = static_cast<unsigned>(nb_writelane(static_cast<int>(x >> 32), l, static_cast<int>(hi)));
 }
 
 // A list entry is the source (x, y, z, G m) in one array and the lanes it
@@ -1582,6 +1580,17 @@ nbody_status bh_resize(BHState* b, size_t n, nbody_sim* s) {
     return NBODY_OK;
 }
 
+nbody_status bh_set_theta(nbody_sim* s) {
+    BHState* b = s->bh;
+    // Before the first allocation there are no thresholds yet: bh_resize
+    // computes them from the handle's theta.
+    if (!b || !b->fixed_root || !b->thr) return NBODY_OK;
+    hipLaunchKernelGGL(open_threshold_kernel, dim3(1), dim3(64), 0, s->stream, b->root, theta_below(s->cfg.theta),
+                       b->thr);
+    NB_HIP(hipGetLastError());
+    return NBODY_OK;
+}
+
 // The eight LSD passes over the keys in tiles of 256 x ITEMS (the last pass
 // also gathers the sorted positions); the sorted buffer is b->sorted_buf.
 template <int ITEMS>
@@ -1664,8 +1673,7 @@ nbody_status bh_step(nbody_sim* s, float dt, bool kick, hipEvent_t ev_b, hipEven
         L.n = n;
         L.n_pad = static_cast<int>(s->n_pad);
         L.law = s->cfg.law;
-        L.soft = static_cast<float>(s->cfg.softening);
-        L.tiny = direct_tiny_for(s->max_gm);
+        L.sc = s->dscale;
         L.dt = dt;
         L.dts = static_cast<float>(static_cast<double>(dt) / s->cfg.scale);
         NB_HIP(direct_step(L, s->shape, kick, st, ev_b, ev_e));

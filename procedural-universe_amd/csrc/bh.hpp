@@ -21,6 +21,10 @@ void bh_destroy(BHState* b);
 nbody_status bh_resize(BHState* b, size_t n, nbody_sim* s);
 // One force evaluation (kick: a full step) of the handle's current state.
 nbody_status bh_step(nbody_sim* s, float dt, bool kick, hipEvent_t ev_force_begin, hipEvent_t ev_force_end);
+// Recompute what depends on the handle's theta (a fixed root's opening
+// thresholds; a dynamic root recomputes them every step).  Enqueued on the
+// handle's stream.
+nbody_status bh_set_theta(nbody_sim* s);
 // Check a fixed root (corner on the root's level-21 grid); out_params may be null.
 nbody_status bh_fixed_root(const nbody_config* c, void* out_params);
 // Counters of the steps since the last reset.

@@ -19,7 +19,9 @@
 // summation are described with the shared core in direct_core.hpp.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
+#include <cmath>
 
 #include "../../include/nbody.h"
 #include "direct.hpp"
@@ -30,51 +32,54 @@ namespace nbody {
 namespace {
 
 using core::kahan_add;
-using core::in_vgpr;
+
+// Kick then drift of target i with the acceleration (ax, ay, az).
+__device__ __forceinline__ void kick_drift(const DirectLaunch& L, int i, float ax, float ay, float az) {
+    float4 v = L.vel[i];
+    float4 x = L.src[i];
+    v.x = __builtin_fmaf(ax, L.dt, v.x);
+    v.y = __builtin_fmaf(ay, L.dt, v.y);
+    v.z = __builtin_fmaf(az, L.dt, v.z);
+    x.x = __builtin_fmaf(v.x, L.dts, x.x);
+    x.y = __builtin_fmaf(v.y, L.dts, x.y);
+    x.z = __builtin_fmaf(v.z, L.dts, x.z);
+    L.vel[i] = v;
+    L.pos_out[i] = x;
+}
 
 // One workgroup: targets [blockIdx.x * BLOCK * K, +BLOCK*K), sources
 // [blockIdx.y * jchunk, +jchunk) clipped to n_pad.  KICK fuses the kick-drift
 // update (only launched with a single j-chunk).
 template <int LAW, int K, int BLOCK, bool KICK>
 __global__ __launch_bounds__(BLOCK) void direct_force_kernel(DirectLaunch L, int jchunk) {
-    __shared__ float4 tile[kTile];
+    __shared__ core::SrcTile tiles[2];
 
     const int tid = threadIdx.x;
     const int ibase = blockIdx.x * (BLOCK * K);
-    const float soft = in_vgpr(L.soft);
-    const float tiny = in_vgpr(L.tiny);
 
     float xi[K], yi[K], zi[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int i = min(ibase + tid + k * BLOCK, L.n_pad - 1);
         const float4 q = L.src[i];
-        xi[k] = q.x;
-        yi[k] = q.y;
-        zi[k] = q.z;
+        xi[k] = q.x * L.sc.inv_len;
+        yi[k] = q.y * L.sc.inv_len;
+        zi[k] = q.z * L.sc.inv_len;
     }
 
     const int j0 = blockIdx.y * jchunk;
     const int j1 = min(j0 + jchunk, L.n_pad);
     float sx[K], sy[K], sz[K];
-    core::block_forces<LAW, K, BLOCK>(tile, L.src, j0, j1, xi, yi, zi, soft, tiny, sx, sy, sz);
+    core::block_forces<LAW, K, BLOCK>(tiles, L.src, j0, j1, L.sc, xi, yi, zi, sx, sy, sz);
 
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int i = ibase + tid + k * BLOCK;
         if constexpr (KICK) {
             if (i < L.n) {
-                float4 v = L.vel[i];
-                float4 x = L.src[i];
-                v.x = __builtin_fmaf(sx[k], L.dt, v.x);
-                v.y = __builtin_fmaf(sy[k], L.dt, v.y);
-                v.z = __builtin_fmaf(sz[k], L.dt, v.z);
-                x.x = __builtin_fmaf(v.x, L.dts, x.x);
-                x.y = __builtin_fmaf(v.y, L.dts, x.y);
-                x.z = __builtin_fmaf(v.z, L.dts, x.z);
-                L.vel[i] = v;
-                L.pos_out[i] = x;
-                L.acc[i] = make_float4(sx[k], sy[k], sz[k], 0.f);
+                const float ax = sx[k] * L.sc.out, ay = sy[k] * L.sc.out, az = sz[k] * L.sc.out;
+                kick_drift(L, i, ax, ay, az);
+                L.acc[i] = make_float4(ax, ay, az, 0.f);
             }
         } else {
             if (i < L.n_pad) L.part[(size_t)blockIdx.y * L.n_pad + i] = make_float4(sx[k], sy[k], sz[k], 0.f);
@@ -83,7 +88,8 @@ __global__ __launch_bounds__(BLOCK) void direct_force_kernel(DirectLaunch L, int
 }
 
 // Adds the j-chunk partial sums of each target in chunk order (compensated),
-// then either kicks and drifts (kick != 0) or only stores the acceleration.
+// scales them back, then either kicks and drifts (kick != 0) or only stores
+// the acceleration.
 __global__ __launch_bounds__(256) void reduce_kick_kernel(DirectLaunch L, int split, int kick) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= L.n) return;
@@ -94,36 +100,24 @@ __global__ __launch_bounds__(256) void reduce_kick_kernel(DirectLaunch L, int sp
         kahan_add(sy, cy, p.y);
         kahan_add(sz, cz, p.z);
     }
-    L.acc[i] = make_float4(sx, sy, sz, 0.f);
-    if (kick) {
-        float4 v = L.vel[i];
-        float4 x = L.src[i];
-        v.x = __builtin_fmaf(sx, L.dt, v.x);
-        v.y = __builtin_fmaf(sy, L.dt, v.y);
-        v.z = __builtin_fmaf(sz, L.dt, v.z);
-        x.x = __builtin_fmaf(v.x, L.dts, x.x);
-        x.y = __builtin_fmaf(v.y, L.dts, x.y);
-        x.z = __builtin_fmaf(v.z, L.dts, x.z);
-        L.vel[i] = v;
-        L.pos_out[i] = x;
-    }
+    const float ax = sx * L.sc.out, ay = sy * L.sc.out, az = sz * L.sc.out;
+    L.acc[i] = make_float4(ax, ay, az, 0.f);
+    if (kick) kick_drift(L, i, ax, ay, az);
 }
 
 // Acceleration at arbitrary points from all sources: one target per lane,
 // every workgroup over the whole source range.
 template <int LAW>
 __global__ __launch_bounds__(256) void field_kernel(const float4* __restrict__ src, int n_src_pad,
-                                                    const float4* __restrict__ tgt, int m, float soft_s, float tiny_s,
+                                                    const float4* __restrict__ tgt, int m, DirectScale sc,
                                                     float4* __restrict__ out) {
-    __shared__ float4 tile[kTile];
+    __shared__ core::SrcTile tiles[2];
     const int i = blockIdx.x * 256 + threadIdx.x;
-    const float soft = in_vgpr(soft_s);
-    const float tiny = in_vgpr(tiny_s);
     const float4 q = tgt[min(i, m - 1)];
-    float xi[1] = {q.x}, yi[1] = {q.y}, zi[1] = {q.z};
+    float xi[1] = {q.x * sc.inv_len}, yi[1] = {q.y * sc.inv_len}, zi[1] = {q.z * sc.inv_len};
     float sx[1], sy[1], sz[1];
-    core::block_forces<LAW, 1, 256>(tile, src, 0, n_src_pad, xi, yi, zi, soft, tiny, sx, sy, sz);
-    if (i < m) out[i] = make_float4(sx[0], sy[0], sz[0], 0.f);
+    core::block_forces<LAW, 1, 256>(tiles, src, 0, n_src_pad, sc, xi, yi, zi, sx, sy, sz);
+    if (i < m) out[i] = make_float4(sx[0] * sc.out, sy[0] * sc.out, sz[0] * sc.out, 0.f);
 }
 
 using ForceFn = void (*)(DirectLaunch, int);
@@ -166,6 +160,31 @@ bool direct_shape_valid(const DirectShape& s) {
     return k_ok && b_ok && s.split >= 1;
 }
 
+DirectScale direct_scale_for(int law, double softening, double extent, double max_gm) {
+    DirectScale sc{};
+    // lambda: the power of two that puts the extent at 32 to 64 scaled units
+    int e_len = 0;
+    if (extent > 0 && std::isfinite(extent)) {
+        (void)std::frexp(extent, &e_len);  // extent in [2^(e-1), 2^e)
+        e_len -= 6;
+    }
+    e_len = std::max(-120, std::min(120, e_len));
+    // c: the power of two at or below the largest G m
+    int e_mass = 0;
+    if (max_gm > 0 && std::isfinite(max_gm)) {
+        (void)std::frexp(max_gm, &e_mass);  // max_gm in [2^(e-1), 2^e)
+        e_mass -= 1;
+    }
+    e_mass = std::max(-120, std::min(120, e_mass));
+    // keep out = 2^(e_mass - 2 e_len) a normal float
+    e_mass = std::max(2 * e_len - 120, std::min(2 * e_len + 120, e_mass));
+    sc.inv_len = std::ldexp(1.0f, -e_len);
+    sc.mass_k = law == NBODY_LAW_R ? std::ldexp(1.0f, e_mass) : std::ldexp(1.0f, -e_mass);
+    sc.soft = static_cast<float>(std::ldexp(softening, -2 * e_len));
+    sc.out = std::ldexp(1.0f, e_mass - 2 * e_len);
+    return sc;
+}
+
 float direct_tiny_for(double max_gm) {
     const double lim = 2.0 * (max_gm > 0 ? max_gm : 0.0) / static_cast<double>(FLT_MAX);
     const double t = 4.0 * lim * lim;
@@ -190,14 +209,14 @@ DirectShape direct_default_shape(int n_pad) {
     return s;
 }
 
-hipError_t direct_field(const float4* src, int n_src_pad, const float4* tgt, int m, int law, float soft, float tiny,
+hipError_t direct_field(const float4* src, int n_src_pad, const float4* tgt, int m, int law, const DirectScale& sc,
                         float4* out, hipStream_t stream) {
     if (m <= 0) return hipSuccess;
     const dim3 grid((m + 255) / 256);
     if (law == NBODY_LAW_R)
-        hipLaunchKernelGGL(field_kernel<NBODY_LAW_R>, grid, dim3(256), 0, stream, src, n_src_pad, tgt, m, soft, tiny, out);
+        hipLaunchKernelGGL(field_kernel<NBODY_LAW_R>, grid, dim3(256), 0, stream, src, n_src_pad, tgt, m, sc, out);
     else
-        hipLaunchKernelGGL(field_kernel<NBODY_LAW_P>, grid, dim3(256), 0, stream, src, n_src_pad, tgt, m, soft, tiny, out);
+        hipLaunchKernelGGL(field_kernel<NBODY_LAW_P>, grid, dim3(256), 0, stream, src, n_src_pad, tgt, m, sc, out);
     return hipGetLastError();
 }
 

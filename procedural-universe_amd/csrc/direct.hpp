@@ -20,24 +20,40 @@ struct DirectShape {
     int split;       // >= 1; > 1 adds a fixed-order reduction pass
 };
 
+// Powers of two in which a system's pair arithmetic runs (exact rescalings):
+// positions are multiplied by inv_len = 1 / lambda, masses enter as
+// c / (G m) (law R) or G m / c (law P), and the sums come back multiplied by
+// out = c / lambda^2.  lambda puts the system's extent at 32 to 64 scaled
+// units, c is the power of two at or below the largest G m.
+struct DirectScale {
+    float inv_len;  // 1 / lambda
+    float mass_k;   // law R: c; law P: 1 / c
+    float soft;     // S / lambda^2 (law R) or eps^2 / lambda^2 (law P)
+    float out;      // c / lambda^2
+};
+
+// extent: the largest coordinate range of the system's positions (0 for a
+// single point); max_gm: its largest |G m| (0 when massless).
+DirectScale direct_scale_for(int law, double softening, double extent, double max_gm);
+
 struct DirectLaunch {
     const float4* src;  // n_pad records (x, y, z, G*m): positions at step start
-    float4* part;       // split > 1 or no kick: partial sums [split][n_pad]
+    float4* part;       // split > 1 or no kick: partial sums [split][n_pad] (scaled units)
     float4* pos_out;    // kick: positions after the step (other buffer)
     float4* vel;        // kick: velocities, updated in place
     float4* acc;        // accelerations of this evaluation (n_pad)
     int n;              // real particles
     int n_pad;          // padded count, multiple of kTile
     int law;            // nbody_law
-    float soft;         // S (law R) or eps^2 (law P)
-    float tiny;         // law R: added to r2 (r2 + S)^2; keeps G m_j / sqrt(tiny) finite
+    DirectScale sc;     // the system's scaled units
     float dt;           // kick step
     float dts;          // drift factor dt / scale
 };
 
 bool direct_shape_valid(const DirectShape& s);
 
-// Smallest `tiny` (>= FLT_MIN) for which max_gm / sqrt(tiny) stays below FLT_MAX / 2.
+// Smallest `tiny` (>= FLT_MIN) for which max_gm / sqrt(tiny) stays below
+// FLT_MAX / 2 (the Barnes-Hut list evaluation's guard).
 float direct_tiny_for(double max_gm);
 
 // Effective number of j-chunks for a shape at this size (never an empty chunk).
@@ -54,8 +70,9 @@ hipError_t direct_step(const DirectLaunch& L, const DirectShape& s, bool kick, h
                        hipEvent_t ev_force_begin, hipEvent_t ev_force_end);
 
 // Accelerations at m points tgt[] (float4, w ignored) from the n_src_pad
-// sources src[] (padded with massless dummies): out[i], float4 (ax, ay, az, 0).
-hipError_t direct_field(const float4* src, int n_src_pad, const float4* tgt, int m, int law, float soft, float tiny,
+// sources src[] (padded with massless dummies) in the sources' scaled units:
+// out[i], float4 (ax, ay, az, 0).
+hipError_t direct_field(const float4* src, int n_src_pad, const float4* tgt, int m, int law, const DirectScale& sc,
                         float4* out, hipStream_t stream);
 
 }  // namespace nbody

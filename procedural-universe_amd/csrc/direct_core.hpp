@@ -1,23 +1,38 @@
 // Device core of the direct sum, shared by the single-system kernel and the
 // batch executor so that a system's forces are the same bits either way.
 //
-// Pair arithmetic (FP32, hardware reciprocal square root), with d = x_j - x_i:
-//   law R (reference, src/Sim/Physics.hpp:25-35):  a += G m_j d / (|d| (|d|^2 + S))
-//            computed as d * (G m_j) * rsq(r2 * (r2 + S)^2 + tiny)
-//   law P (Plummer):                              a += G m_j d / (|d|^2 + eps^2)^(3/2)
-// For law R, `tiny` keeps a coincident pair (r2 == 0, including j == i) finite:
-// the factor becomes G m_j / sqrt(tiny), which the host keeps below FLT_MAX, so
-// d = 0 makes the term exactly 0, as the reference's Normalize of a zero vector
-// does; for r2 > 0 it changes the factor by tiny / q (below 1e-7 unless
-// q < 1e-30, i.e. pairs far closer than FP32 positions of order 1 resolve).
+// Scaled units.  Each system carries two powers of two (DirectScale): a length
+// lambda and a mass scale c.  Sources and targets are multiplied by 1/lambda
+// when they are loaded (exact: a power of two), so d' = d / lambda exactly and
+// every rounding of the pair arithmetic is the unscaled one with a shifted
+// exponent; lambda is chosen from the system's extent so that the squared
+// distances sit in the middle of FP32's range whatever the units.  The sums
+// are multiplied by out = c / lambda^2 at the end (exact again).
+//
+// Law R (reference, src/Sim/Physics.hpp:25-35), per source j staged in LDS as
+// (x', y', z', h) and k with h = c / (G m_j), k = S' h (S' = S / lambda^2):
+//     r2 = |d'|^2 + tiny0,  t = fma(r2, h, k) = (r2 + S') c / (G m_j),
+//     q  = r2 t^2 + tiny,   a' += d' * rsq(q)
+// so rsq(q) = (G m_j / c) / (|d'| (r2 + S')): the source's mass is folded into
+// the reciprocal square root and the pair costs 12 add/sub/mul/fma and one
+// v_rsq_f32 (the unfolded form needs a 13th multiply by G m_j).  A massless
+// source (the padding dummies) has h = k = +inf: t, q = +inf and rsq = 0
+// exactly; tiny0 = FLT_MIN keeps r2 > 0 so that no 0 * inf appears.  tiny
+// keeps a coincident pair (j == i included) finite: q >= tiny, so d' = 0
+// makes the term exactly 0, as the reference's Normalize of a zero vector
+// does; for r2 > 0 it changes q by tiny / q (nothing above q ~ 1e-31).
+// Masses must be >= 0 (the square loses a negative mass's sign).
+//
+// Law P (Plummer), per source (x', y', z', w) with w = G m_j / c:
+//     a' += d' * w * rsq(r2 + eps'^2)^3.
 //
 // Issue rates on gfx950 shape the arithmetic: FP32 add/sub/mul/fma whose
 // operands are all VGPRs issue at twice the rate of the same instruction with
 // an SGPR or literal operand, or of v_min/v_max, and v_rsq_f32 issues at half
 // that single rate.  So every per-pair operation is a VGPR-only add/sub/mul/fma
-// (12 of them) plus one v_rsq_f32: the softening and `tiny` are copied into
-// VGPRs once, the coincident-pair guard is an FMA rather than a clamp, and the
-// rsq of four pairs are issued together.
+// plus one v_rsq_f32: the constants are copied into VGPRs once, the guards are
+// FMA operands rather than clamps, and the rsq of four pairs are issued
+// together.
 //
 // Summation: per LDS tile of kTile sources each target sums in plain FP32
 // registers; the tile sums are added into a compensated (Kahan) accumulator,
@@ -107,18 +122,48 @@ __device__ __forceinline__ void pair_acc(const float gm, const float inv, const 
     az = __builtin_fmaf(dz, s, az);
 }
 
-// Sums of the accelerations on the K targets (xi, yi, zi) of this lane from
-// the sources src[j0, j1) (j1 - j0 a multiple of kTile), streamed through the
-// workgroup's LDS tile; the next tile is prefetched into registers while the
-// current one is consumed.  Every thread of the workgroup must call it.
+// One staged source tile: positions in scaled units with the law's per-source
+// factor (h for law R, w for law P) in .w, and law R's k = S' h in a second
+// array (so that four sources' k come in one 16-byte read).
+struct SrcTile {
+    float4 p[kTile];
+    float k[kTile];
+};
+
+// The staged record of a source (x, y, z, G m) of this system.
+template <int LAW>
+__device__ __forceinline__ void stage_source(const float4 s, const DirectScale& sc, float4& p, float& k) {
+    p.x = s.x * sc.inv_len;
+    p.y = s.y * sc.inv_len;
+    p.z = s.z * sc.inv_len;
+    if constexpr (LAW == NBODY_LAW_R) {
+        const bool live = s.w > 0.f;
+        const float h = live ? __fdiv_rn(sc.mass_k, s.w) : __builtin_inff();
+        p.w = h;
+        k = live ? sc.soft * h : __builtin_inff();
+    } else {
+        p.w = s.w * sc.mass_k;
+        k = 0.f;
+    }
+}
+
+// Sums of the accelerations on the K targets (xi, yi, zi) of this lane, in
+// scaled units (multiply by sc.out), from the sources src[j0, j1) (j1 - j0 a
+// multiple of kTile).  The tiles are staged through the workgroup's two LDS
+// buffers in turn: the next tile is loaded from memory into registers while
+// the current one is consumed, and written to the other buffer, so one
+// barrier per tile separates the writes from the reads.  Every thread of the
+// workgroup must call it; targets are already in scaled units.
 template <int LAW, int K, int BLOCK>
-__device__ __forceinline__ void block_forces(float4* tile, const float4* __restrict__ src, const int j0, const int j1,
-                                             const float (&xi)[K], const float (&yi)[K], const float (&zi)[K],
-                                             const float soft, const float tiny, float (&sx)[K], float (&sy)[K],
-                                             float (&sz)[K]) {
+__device__ __forceinline__ void block_forces(SrcTile* tiles, const float4* __restrict__ src, const int j0, const int j1,
+                                             const DirectScale& sc, const float (&xi)[K], const float (&yi)[K],
+                                             const float (&zi)[K], float (&sx)[K], float (&sy)[K], float (&sz)[K]) {
     static_assert(kTile % BLOCK == 0, "tile must be a multiple of the block");
     constexpr int PER = kTile / BLOCK;
     const int tid = threadIdx.x;
+    const float tiny0 = in_vgpr(__FLT_MIN__);
+    const float tiny = in_vgpr(__FLT_MIN__);
+    const float soft = in_vgpr(sc.soft);
     float cx[K], cy[K], cz[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -128,16 +173,18 @@ __device__ __forceinline__ void block_forces(float4* tile, const float4* __restr
     float4 pre[PER];
 #pragma unroll
     for (int l = 0; l < PER; ++l) pre[l] = src[j0 + tid + l * BLOCK];
-
-    for (int jt = j0; jt < j1; jt += kTile) {
-        __syncthreads();
 #pragma unroll
-        for (int l = 0; l < PER; ++l) tile[tid + l * BLOCK] = pre[l];
-        __syncthreads();
-        if (jt + kTile < j1) {
+    for (int l = 0; l < PER; ++l) stage_source<LAW>(pre[l], sc, tiles[0].p[tid + l * BLOCK], tiles[0].k[tid + l * BLOCK]);
+    __syncthreads();
+
+    int buf = 0;
+    for (int jt = j0; jt < j1; jt += kTile) {
+        const bool more = jt + kTile < j1;
+        if (more) {
 #pragma unroll
             for (int l = 0; l < PER; ++l) pre[l] = src[jt + kTile + tid + l * BLOCK];
         }
+        const SrcTile& T = tiles[buf];
 
         float ax[K], ay[K], az[K];
 #pragma unroll
@@ -149,15 +196,39 @@ __device__ __forceinline__ void block_forces(float4* tile, const float4* __restr
         for (int j = 0; j < kTile; j += 4) {
             float4 pj[4];
 #pragma unroll
-            for (int g = 0; g < 4; ++g) pj[g] = tile[j + g];
+            for (int g = 0; g < 4; ++g) pj[g] = T.p[j + g];
+            const float4 kj = *reinterpret_cast<const float4*>(&T.k[j]);
+            const float kg[4] = {kj.x, kj.y, kj.z, kj.w};
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 float dx[4], dy[4], dz[4], q[4];
 #pragma unroll
-                for (int g = 0; g < 4; ++g) q[g] = pair_arg<LAW>(pj[g], xi[k], yi[k], zi[k], soft, tiny, dx[g], dy[g], dz[g]);
+                for (int g = 0; g < 4; ++g) {
+                    dx[g] = pj[g].x - xi[k];
+                    dy[g] = pj[g].y - yi[k];
+                    dz[g] = pj[g].z - zi[k];
+                    if constexpr (LAW == NBODY_LAW_R) {
+                        const float r2 = __builtin_fmaf(dz[g], dz[g], __builtin_fmaf(dy[g], dy[g], __builtin_fmaf(dx[g], dx[g], tiny0)));
+                        const float t = __builtin_fmaf(r2, pj[g].w, kg[g]);
+                        const float u = r2 * t;
+                        q[g] = __builtin_fmaf(u, t, tiny);
+                    } else {
+                        q[g] = __builtin_fmaf(dz[g], dz[g], __builtin_fmaf(dy[g], dy[g], __builtin_fmaf(dx[g], dx[g], soft)));
+                    }
+                }
                 rsq4(q[0], q[1], q[2], q[3]);
 #pragma unroll
-                for (int g = 0; g < 4; ++g) pair_acc<LAW>(pj[g].w, q[g], dx[g], dy[g], dz[g], ax[k], ay[k], az[k]);
+                for (int g = 0; g < 4; ++g) {
+                    float f;
+                    if constexpr (LAW == NBODY_LAW_R) {
+                        f = q[g];
+                    } else {
+                        f = (pj[g].w * q[g]) * (q[g] * q[g]);
+                    }
+                    ax[k] = __builtin_fmaf(dx[g], f, ax[k]);
+                    ay[k] = __builtin_fmaf(dy[g], f, ay[k]);
+                    az[k] = __builtin_fmaf(dz[g], f, az[k]);
+                }
             }
         }
 #pragma unroll
@@ -166,8 +237,15 @@ __device__ __forceinline__ void block_forces(float4* tile, const float4* __restr
             kahan_add(sy[k], cy[k], ay[k]);
             kahan_add(sz[k], cz[k], az[k]);
         }
+        if (more) {
+            // The other buffer was last read before the previous barrier.
+#pragma unroll
+            for (int l = 0; l < PER; ++l)
+                stage_source<LAW>(pre[l], sc, tiles[buf ^ 1].p[tid + l * BLOCK], tiles[buf ^ 1].k[tid + l * BLOCK]);
+            __syncthreads();
+            buf ^= 1;
+        }
     }
-
 }
 
 }  // namespace core

@@ -38,6 +38,7 @@ struct nbody_sim {
     size_t diag_elems = 0;
     std::vector<double> mass_host;        // FP64 masses (FORCES read-back = m * a)
     double max_gm = 0.0;                  // largest |G*m| (sizes the law-R guard)
+    nbody::DirectScale dscale{1.f, 1.f, 0.f, 1.f};  // direct sum's scaled units
     bool acc_valid = false;
 
     nbody::DirectShape shape{4, 256, 1};
