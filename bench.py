@@ -175,6 +175,42 @@ def bench_batch(pu, args):
     }
 
 
+def batch_step_latency(pu, steps=50, seed=0x5EED0004, eps=0.01, dt=1e-3, count=512):
+    """Per-step latency of the batched workload: 512 galaxies (N log-uniform in
+    [1024, 65536], Plummer or exponential disc), autotuned, then `steps` single
+    steps, each timed on the batch's stream by the HIP events around its graph
+    launch (nbody_batch_get_timing).  Returns p50/p99 of those step times and
+    the aggregate interactions/s they give."""
+    gal = pu.batch_galaxies(count, seed, 1024, 65536)
+    cap = sum((len(m) + 255) // 256 * 256 for (_, _, m) in gal)
+    cfg = pu.default_config(law=pu.LAW_R, softening=eps * eps)
+    with pu.Batch(cfg, cap) as b:
+        for p, v, m in gal:
+            b.add(p, v, m)
+        per_step = float(b.info().interactions_per_step)
+        tuning = b.tune(dt)
+        b.step(dt, 2)  # both parities' graphs captured before timing
+        b.synchronize()
+        ms = []
+        for _ in range(steps):
+            b.step(dt, 1)
+            ms.append(b.timing().step_ms)
+    ms = np.array(ms)
+    return {
+        "workload": f"{count} galaxies, N log-uniform in [1024, 65536] (Plummer or exponential disc, 50/50), "
+        f"direct-sum FP32, eps={eps}, dt={dt}, one graph launch per step",
+        "steps": steps,
+        "p50_ms": float(np.percentile(ms, 50)),
+        "p99_ms": float(np.percentile(ms, 99)),
+        "mean_ms": float(ms.mean()),
+        "max_ms": float(ms.max()),
+        "interactions_per_step": per_step,
+        "interactions_per_s": per_step / (ms.mean() * 1e-3),
+        "timing": "HIP events around each step's graph launch on the batch's stream",
+        "tuning": tuning,
+    }
+
+
 def bench_bh(pu, args):
     """Barnes-Hut, theta = 0.5, monopoles: two exponential-disc galaxies of
     N/2 particles on a parabolic encounter (separation 40, pericentre 5, second
@@ -478,6 +514,7 @@ def main():
     ap.add_argument("--subsample", type=int, default=65536)
     ap.add_argument("--galaxies", type=int, default=512)
     ap.add_argument("--baseline-steps", type=int, default=10)
+    ap.add_argument("--no-batched", action="store_true", help="direct: skip the batched step-latency leg")
     args = ap.parse_args()
 
     if args.workload == "copy":
@@ -622,6 +659,11 @@ def main():
             "steps": args.warmup + args.steps,
             "note": "kick-drift velocities are half a step from positions (reference K-D form)",
         }
+
+    sim.close()
+    if not args.no_batched and n == 1 << 20:
+        # the metric's second half: batched per-step latency (512 galaxies)
+        result["batched"] = batch_step_latency(pu)
 
     if not args.no_cpu_baseline:
         # CPU-baseline leg: the CPU restatement is the checker of the first

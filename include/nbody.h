@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define NBODY_ABI_VERSION 1
+#define NBODY_ABI_VERSION 2
 
 /* ------------------------------------------------------------------------ */
 /* Particle record                                                           */
@@ -270,7 +270,8 @@ typedef struct nbody_bh_stats {
     uint64_t cells_opened;    /* cells whose children a wavefront examined                       */
 } nbody_bh_stats;
 
-/* Counters of the Barnes-Hut steps since the last nbody_step began. */
+/* Counters of the Barnes-Hut steps since the last nbody_step began, or of the
+ * force evaluation of the last nbody_compute_accelerations if that came later. */
 nbody_status nbody_bh_stats_get(nbody_sim* sim, nbody_bh_stats* out);
 
 /* Sorted 63-bit Morton keys and the particle order (stable) of the last tree
@@ -397,7 +398,9 @@ typedef struct nbody_server_config {
     float    dt;               /* step of every request                                    */
     int32_t  keep_results;     /* keep final states for nbody_server_fetch                 */
     int32_t  bh_steps_ahead;   /* Barnes-Hut steps enqueued per request between checks (0: 1) */
-    int32_t  reserved[7];
+    int32_t  inject_alloc_failures; /* test hook: the server's next k request allocations fail
+                                       as a full device would (0 in production)               */
+    int32_t  reserved[6];
 } nbody_server_config;
 
 typedef struct nbody_request_result {
@@ -425,6 +428,8 @@ typedef struct nbody_server_stats {
     uint64_t interactions;     /* of completed requests                           */
     uint64_t ticks;            /* ticks that completed or enqueued work            */
     double   busy_seconds;     /* time spent in those ticks                       */
+    uint64_t alloc_deferrals;  /* admissions put back because a device allocation
+                                  failed (retried on later ticks)                  */
 } nbody_server_stats;
 
 /* law R, eps = 0.02, theta = 0.6, 64 GiB budget, 64 Mi-particle arena,
@@ -447,6 +452,9 @@ nbody_status nbody_server_submit_plummer(nbody_server* server, uint64_t seed, si
  * steps that have completed (the batch's last step, each Barnes-Hut handle's
  * enqueued steps), retire finished requests, admit queued ones that fit, and
  * enqueue the next batch step and the next Barnes-Hut steps of idle handles.
+ * A request whose device allocation fails (NBODY_ERR_OUT_OF_MEMORY) releases
+ * the idle handles and stays queued, in its place, for a later tick; the tick
+ * itself does not fail.
  * Each request advances as soon as its own work completes.  *completed =
  * requests that finished in this tick.  Call it in a loop; a tick that finds
  * nothing to do sleeps for about 100 microseconds. */

@@ -208,7 +208,8 @@ class ServerConfig(ctypes.Structure):
         ("dt", ctypes.c_float),
         ("keep_results", ctypes.c_int32),
         ("bh_steps_ahead", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("inject_alloc_failures", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -240,6 +241,7 @@ class ServerStats(ctypes.Structure):
         ("interactions", ctypes.c_uint64),
         ("ticks", ctypes.c_uint64),
         ("busy_seconds", ctypes.c_double),
+        ("alloc_deferrals", ctypes.c_uint64),
     ]
 
     def as_dict(self):

@@ -9,9 +9,11 @@
 //                   [--seed S]
 //
 // The particle set comes from one of the reference's seeders drawn on the
-// device (nbody_init_seeded).  Both brute-force entries run the device direct
-// sum, as CreateNBodySim maps them here; the frame time includes the
-// read-back of the records, which the reference's Update also performs.
+// device (nbody_init_seeded).  The library has no CPU simulator: both
+// brute-force entries run the device direct sum, as CreateNBodySim maps them
+// here, and the second column says so; the "Brute Force CPU" row is not a
+// CPU timing.  The frame time includes the read-back of the records, which
+// the reference's Update also performs.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -81,7 +83,8 @@ int main(int argc, char** argv) {
             for (int f = 0; f < frames; ++f) sim->Update(1.0f);
             const double ms =
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / frames;
-            std::printf("%-16s %12.3f ms/frame\n", NBodySimGetName(type).c_str(), ms);
+            const char* ran = type == ENBodySim::BarnesHut ? "device Barnes-Hut" : "device direct sum";
+            std::printf("%-16s %-18s %12.3f ms/frame\n", NBodySimGetName(type).c_str(), ran, ms);
         }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "[Error] %s\n", e.what());

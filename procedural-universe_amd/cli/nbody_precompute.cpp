@@ -8,11 +8,13 @@
 //                    [--method bh|direct] [--steps K] [--out path] [--seed S]
 //
 // As in the reference, the step is timestep / 60 and the simulator is
-// Barnes-Hut with the reference constants.  A loaded file (-f) is used as the
-// initial state (the reference overwrites it with its seeder,
-// SimulationState.cpp:283-289); without one, particles are uniform in
-// [-500, 500)^3 with masses uniform in [1e20, 1e30), the ranges of the
-// reference's RandomSeeder (src/Sim/RandomSeeder.cpp:16-19).
+// Barnes-Hut with the reference constants.  Without a file the particles come
+// from the reference's StarSystem seeder (SimulationState.cpp:286-290 seeds
+// with EParticleSeeder::StarSystem; src/Sim/StarSystemSeeder.cpp:17-55), drawn
+// on the device by nbody_init_seeded (--seeder picks another of the
+// reference's seeders).  A loaded file (-f) is used as the initial state (the
+// reference overwrites it with its seeder, SimulationState.cpp:283-289, a
+// defect not reproduced).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -26,13 +28,14 @@
 static void usage() {
     std::printf(
         "usage: nbody_precompute [-t simtime_s] [-s timestep] [-p particles] [-f file.nbody]\n"
-        "                        [--method bh|direct] [--steps K] [--out path] [--seed S]\n");
+        "                        [--method bh|direct] [--steps K] [--out path] [--seed S]\n"
+        "                        [--seeder star|random|galaxy]\n");
 }
 
 int main(int argc, char** argv) {
     int simtime = 10, particles = 4000, steps = -1;
     float timestep = 0.02f;
-    std::string file, out, method = "bh";
+    std::string file, out, method = "bh", seeder = "star";
     unsigned long long seed = 0x5EED0006ULL;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -52,6 +55,7 @@ int main(int argc, char** argv) {
         else if (a == "--steps") steps = std::atoi(next());
         else if (a == "--out") out = next();
         else if (a == "--seed") seed = std::strtoull(next(), nullptr, 0);
+        else if (a == "--seeder") seeder = next();
         else if (a == "-h" || a == "--help") {
             usage();
             return 0;
@@ -60,11 +64,13 @@ int main(int argc, char** argv) {
             return 2;
         }
     }
-    if (method != "bh" && method != "direct") {
+    if ((method != "bh" && method != "direct") || (seeder != "star" && seeder != "random" && seeder != "galaxy")) {
         usage();
         return 2;
     }
     try {
+        const ENBodySim kind = method == "bh" ? ENBodySim::BarnesHut : ENBodySim::BruteForceGPU;
+        auto sim = CreateNBodySim(kind);
         std::vector<Particle> p;
         if (!file.empty()) {
             size_t n = 0;
@@ -74,18 +80,17 @@ int main(int argc, char** argv) {
             if (st != NBODY_OK) throw nbody_error("nbody_snapshot_read", st);
             std::printf("[Info] Read %zu particles from file\n", n);
         } else {
-            std::vector<float> pos(3 * static_cast<size_t>(particles)), vel(3 * static_cast<size_t>(particles));
-            std::vector<double> mass(particles);
-            nbody_status st = nbody_ic_uniform(seed, particles, 500.0, 1e20, 1e30, pos.data(), vel.data(), mass.data());
-            if (st != NBODY_OK) throw nbody_error("nbody_ic_uniform", st);
+            // CreateParticleSeeder(particles, EParticleSeeder::StarSystem)->Seed()
+            nbody_sim* h = static_cast<DeviceNBodySim*>(sim.get())->handle();
+            const int32_t kind_id = seeder == "star" ? NBODY_SEEDER_STAR_SYSTEM
+                                    : seeder == "random" ? NBODY_SEEDER_RANDOM : NBODY_SEEDER_GALAXY;
+            nbody_status st = nbody_init_seeded(h, kind_id, seed, static_cast<size_t>(particles), 1.0);
+            if (st != NBODY_OK) throw nbody_error("nbody_init_seeded", st);
             p.assign(particles, Particle{});
-            for (int i = 0; i < particles; ++i) {
-                for (int a = 0; a < 3; ++a) p[i].position[a] = pos[3 * i + a];
-                p[i].mass = mass[i];
-            }
+            st = nbody_read(h, p.data(), p.size(), NBODY_FIELD_POSITION | NBODY_FIELD_VELOCITY | NBODY_FIELD_MASS);
+            if (st != NBODY_OK) throw nbody_error("nbody_read", st);
         }
-        auto sim = CreateNBodySim(method == "bh" ? ENBodySim::BarnesHut : ENBodySim::BruteForceGPU);
-        sim->Init(p);
+        sim->Init(p);  // the same state, now bound to the caller's records as the reference's Init does
         const float dt = timestep * (1.0f / 60.0f);  // src/App/NBody.cpp:87
         const auto t0 = std::chrono::steady_clock::now();
         auto tick = t0;
@@ -111,8 +116,7 @@ int main(int argc, char** argv) {
         nbody_status st = nbody_snapshot_write(out.c_str(), p.data(), p.size());
         if (st != NBODY_OK) throw nbody_error("nbody_snapshot_write", st);
         std::printf("[Info] %ld iterations of %zu particles (%s) written to %s\n", iterations, p.size(),
-                    NBodySimGetName(method == "bh" ? ENBodySim::BarnesHut : ENBodySim::BruteForceGPU).c_str(),
-                    out.c_str());
+                    NBodySimGetName(kind).c_str(), out.c_str());
     } catch (const std::exception& e) {
         std::fprintf(stderr, "[Error] %s\n", e.what());
         return 1;

@@ -601,6 +601,7 @@ nbody_status nbody_compute_accelerations(nbody_sim* s, float* acc, size_t n) {
     if (n == 0) return NBODY_OK;
     NB_HIP(hipSetDevice(s->cfg.device));
     s->step_interactions = 0;
+    if (s->bh) NB_TRY(bh_reset_counters(s->bh, s->stream));  // the stats then describe this evaluation
     NB_TRY(force_pass(s, 0.f, false, nullptr, nullptr));
     return nbody_read_soa(s, nullptr, nullptr, acc, n);
 }

@@ -91,8 +91,7 @@ __global__ __launch_bounds__(BLOCK) void batch_step_kernel(BatchLaunch B, unsign
         }
         float sx[K], sy[K], sz[K];
         core::block_forces<LAW, K, BLOCK>(tiles, src, 0, d.n_pad, d.sc, xi, yi, zi, sx, sy, sz);
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
+        for (int k = 0; k < K; ++k) {  // K is 1 or 2: fully unrolled without a pragma
             const int i = w.ibase + tid + k * BLOCK;
             if (i < d.n) {
                 const int r = d.off + i;

@@ -62,6 +62,12 @@
 #include "direct_core.hpp"
 #include "sim.hpp"
 
+// v_writelane_b32 through the compiler's own intrinsic (this clang has no
+// __builtin_amdgcn_writelane): the compiler places the lane select in M0,
+// which it reserves and sets itself, and inserts the wait states its hazard
+// model requires between the ballot that produces the value and the write.
+__device__ int nb_writelane(int value, int lane, int old) __asm("llvm.amdgcn.writelane");
+
 namespace nbody {
 
 namespace {
@@ -1045,12 +1051,6 @@ constexpr int kListCap = 96;
 __device__ __forceinline__ float lane_gm(float gm, unsigned w, int lane) {
     return __int_as_float(__float_as_int(gm) & __builtin_amdgcn_sbfe(static_cast<int>(w), lane & 31, 1));
 }
-
-// v_writelane_b32 through the compiler's own intrinsic (this clang has no
-// __builtin_amdgcn_writelane): the compiler places the lane select in M0,
-// which it reserves and sets itself, and inserts the wait states its hazard
-// model requires between the ballot that produces the value and the write.
-__device__ int nb_writelane(int value, int lane, int old) __asm("llvm.amdgcn.writelane");
 
 // (lo, hi) = x in lane l only (register writes; the other lanes keep lo and
 // hi).  l and x are uniform over the wave.

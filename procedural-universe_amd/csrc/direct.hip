@@ -162,10 +162,12 @@ bool direct_shape_valid(const DirectShape& s) {
 
 DirectScale direct_scale_for(int law, double softening, double extent, double max_gm) {
     DirectScale sc{};
-    // lambda: the power of two that puts the extent at 32 to 64 scaled units
+    // lambda: the power of two that puts the larger of the extent and the
+    // softening length at 32 to 64 scaled units
+    const double len = std::max(extent, std::sqrt(std::max(softening, 0.0)));
     int e_len = 0;
-    if (extent > 0 && std::isfinite(extent)) {
-        (void)std::frexp(extent, &e_len);  // extent in [2^(e-1), 2^e)
+    if (len > 0 && std::isfinite(len)) {
+        (void)std::frexp(len, &e_len);  // len in [2^(e-1), 2^e)
         e_len -= 6;
     }
     e_len = std::max(-120, std::min(120, e_len));

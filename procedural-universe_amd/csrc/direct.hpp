@@ -23,8 +23,12 @@ struct DirectShape {
 // Powers of two in which a system's pair arithmetic runs (exact rescalings):
 // positions are multiplied by inv_len = 1 / lambda, masses enter as
 // c / (G m) (law R) or G m / c (law P), and the sums come back multiplied by
-// out = c / lambda^2.  lambda puts the system's extent at 32 to 64 scaled
-// units, c is the power of two at or below the largest G m.
+// out = c / lambda^2.  lambda puts the larger of the system's extent and its
+// softening length (sqrt S or eps) at 32 to 64 scaled units, c is the power of
+// two at or below the largest G m.  Law R's rsq argument r'^2 (r'^2 + S')^2
+// (c / G m_j)^2 then stays inside FP32's normal range for every pair of the
+// system (and for systems that later grow up to ~1e4 times) unless a source
+// is lighter than about 1e-11 of the heaviest, whose pull is then dropped.
 struct DirectScale {
     float inv_len;  // 1 / lambda
     float mass_k;   // law R: c; law P: 1 / c

@@ -90,6 +90,7 @@ struct nbody_server {
     std::map<uint64_t, std::vector<float>> kept;  // id -> pos[3n] then vel[3n]
     uint64_t next_id = 1;
     bool batch_inflight = false;
+    int32_t inject_left = 0;  // injected allocation failures still to come
     size_t mem_used = 0, mem_peak = 0;
     nbody_server_stats stats{};
 
@@ -118,6 +119,27 @@ void release_idle(nbody_server* s, size_t need) {
     }
 }
 
+// A device allocation for a request, or the injected failure that stands in
+// for one (nbody_server_config.inject_alloc_failures).
+template <class F>
+nbody_status allocate(nbody_server* s, F&& call) {
+    if (s->inject_left > 0) {
+        --s->inject_left;
+        set_error("injected allocation failure");
+        return NBODY_ERR_OUT_OF_MEMORY;
+    }
+    return call();
+}
+
+// The device is fuller than the footprint model says: give back the idle
+// handles' memory and leave the request queued for a later tick.
+nbody_status defer(nbody_server* s) {
+    release_idle(s, s->cfg.memory_budget + 1);
+    s->stats.alloc_deferrals += 1;
+    clear_error();
+    return NBODY_OK;
+}
+
 nbody_status admit(nbody_server* s, Request& r, bool* ok) {
     *ok = false;
     if (r.method == NBODY_METHOD_DIRECT) {
@@ -125,8 +147,10 @@ nbody_status admit(nbody_server* s, Request& r, bool* ok) {
         if (s->mem_used + f > s->cfg.memory_budget) release_idle(s, f);
         if (s->mem_used + f > s->cfg.memory_budget) return NBODY_OK;
         int32_t id = -1;
-        nbody_status st = nbody_batch_add_plummer(s->batch, r.seed, r.n, r.a, r.mass, &id);
+        nbody_status st =
+            allocate(s, [&] { return nbody_batch_add_plummer(s->batch, r.seed, r.n, r.a, r.mass, &id); });
         if (st == NBODY_ERR_CAPACITY) return NBODY_OK;  // arena fragmented or full: wait
+        if (st == NBODY_ERR_OUT_OF_MEMORY) return defer(s);
         NB_TRY(st);
         r.batch_id = id;
         r.footprint = f;
@@ -150,7 +174,9 @@ nbody_status admit(nbody_server* s, Request& r, bool* ok) {
             if (s->mem_used + f > s->cfg.memory_budget) return NBODY_OK;
             nbody_config c = s->cfg.sim;
             c.method = NBODY_METHOD_BARNES_HUT;
-            NB_TRY(nbody_create(&c, &r.sim));
+            const nbody_status sc = allocate(s, [&] { return nbody_create(&c, &r.sim); });
+            if (sc == NBODY_ERR_OUT_OF_MEMORY) return defer(s);
+            NB_TRY(sc);
             // one graph launch per step instead of about forty kernels
             const nbody_status sg = nbody_set_graph(r.sim, 1);
             if (sg != NBODY_OK) {
@@ -164,11 +190,15 @@ nbody_status admit(nbody_server* s, Request& r, bool* ok) {
             s->mem_used += f;
         }
         r.footprint = footprint(s, r.sim_cap, r.method);
-        nbody_status st = nbody_init_plummer(r.sim, r.seed, r.n, r.a, r.mass);
+        // a new handle allocates its state here
+        nbody_status st = allocate(s, [&] { return nbody_init_plummer(r.sim, r.seed, r.n, r.a, r.mass); });
         if (st != NBODY_OK) {
+            const std::string msg = nbody_last_error();  // nbody_destroy clears it
             nbody_destroy(r.sim);
             s->mem_used -= r.footprint;
             r.sim = nullptr;
+            if (st == NBODY_ERR_OUT_OF_MEMORY) return defer(s);
+            set_error("%s", msg.c_str());
             return st;
         }
     }
@@ -243,6 +273,7 @@ nbody_status nbody_server_create(const nbody_server_config* cfg, nbody_server** 
     nbody_server* s = new (std::nothrow) nbody_server();
     NB_CHECK(s, NBODY_ERR_OUT_OF_MEMORY, "host allocation failed");
     s->cfg = *cfg;
+    s->inject_left = std::max(0, cfg->inject_alloc_failures);
     s->t0 = std::chrono::steady_clock::now();
     nbody_status st = nbody_batch_create(&c, cfg->direct_capacity, &s->batch);
     if (st != NBODY_OK) {

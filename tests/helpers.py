@@ -220,3 +220,19 @@ def law_r_force(p, q, mp, mq, G=REF_G, S=REF_S):
     d = np.asarray(q, np.float64) - np.asarray(p, np.float64)
     r = np.linalg.norm(d)
     return G * mp * mq / (r * r + S) * d / r
+
+
+def bh_step_dynamic(oracle, pos, vel, mass, dt, nsteps, theta, G=1.0, S=1e-4, scale=1.0, com_fp64=True):
+    """BarnesHut::Update (src/Sim/BarnesHut.cpp:44-96) repeated nsteps times
+    with the device's dynamic root (the bounding box grown to a power-of-two
+    cube, recomputed every step) instead of the reference's fixed one: the
+    oracle's octree forces, then v += F/m dt in FP64 and x += (float)(v dt/scale)."""
+    p = np.ascontiguousarray(pos, dtype=np.float32).copy()
+    v = np.asarray(vel, dtype=np.float64).copy()
+    m = np.asarray(mass, dtype=np.float64)
+    for _ in range(int(nsteps)):
+        lo, size = oracle.dynamic_root(p)
+        F, _ = oracle.bh_forces(p, m, theta, lo, size, G=G, S=S, com_fp64=com_fp64)
+        v += F / m[:, None] * np.float64(np.float32(dt))
+        p = (p.astype(np.float32) + (v * np.float64(np.float32(dt)) / scale).astype(np.float32)).astype(np.float32)
+    return p, v

@@ -49,7 +49,7 @@ def test_particle_record_layout(pu):
 
 def test_names_and_versions(pu):
     lib = pu.load_library()
-    assert lib.nbody_abi_version() == 1
+    assert lib.nbody_abi_version() == 2
     assert lib.nbody_method_name(pu.METHOD_DIRECT) == b"Brute Force GPU"
     assert lib.nbody_method_name(pu.METHOD_BARNES_HUT) == b"Barnes-Hut"
     assert lib.nbody_method_name(99) == b"Unknown"

@@ -27,9 +27,9 @@ def test_device_plummer_matches_host_generator(gpu):
     assert np.array_equal(pb, p) and np.array_equal(vb, v)
 
 
-def _serve(pu, requests, budget=1 << 34, keep=True, threshold=3000):
+def _serve(pu, requests, budget=1 << 34, keep=True, threshold=3000, inject=0):
     cfg = pu.server_config(memory_budget=budget, direct_capacity=1 << 20, bh_threshold=threshold, keep_results=1,
-                           dt=1e-3)
+                           dt=1e-3, inject_alloc_failures=inject)
     out = {}
     with pu.Server(cfg) as srv:
         ids = [srv.submit_plummer(n, steps, seed) for (n, steps, seed) in requests]
@@ -139,3 +139,21 @@ def test_graph_replays_match_plain_launches(gpu, method):
             assert np.array_equal(a_eager, a_graph), rep
         if method == "bh":
             assert graph.bh_stats()["interactions"] > 0
+
+
+def test_server_defers_requests_whose_allocation_fails(gpu):
+    """Device allocations that fail (forced from the host: the server's next
+    five request allocations report NBODY_ERR_OUT_OF_MEMORY) put the request
+    back in the queue instead of failing the tick; every request still
+    completes, with the same bits as without the failures."""
+    pu = gpu
+    reqs = [(2000, 3, 41), (6000, 2, 42), (1500, 4, 43), (8000, 2, 44), (2500, 3, 45), (5000, 2, 46)]
+    ids0, out0, _, _, st0 = _serve(pu, reqs, threshold=4000)
+    ids1, out1, res1, _, st1 = _serve(pu, reqs, threshold=4000, inject=5)
+    assert st0["alloc_deferrals"] == 0
+    assert st1["alloc_deferrals"] == 5
+    assert st1["completed"] == len(reqs) and st1["pending"] == 0
+    assert {r["method"] for r in res1} == {pu.METHOD_DIRECT, pu.METHOD_BARNES_HUT}
+    for a, b in zip(ids0, ids1):
+        for x, y in zip(out0[a], out1[b]):
+            assert np.array_equal(x, y)

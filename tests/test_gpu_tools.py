@@ -46,23 +46,34 @@ def test_cpp_adapter_update_matches_oracle(gpu, oracle, tmp_path, kind):
     assert rel_err_rows(vals[:, :3], p_ref).max() < 1e-4
 
 
-def test_precompute_driver_writes_snapshot(gpu, tmp_path):
+def test_precompute_driver_writes_snapshot(gpu, oracle, tmp_path):
+    """`-c` seeds with the StarSystem seeder as RunSimulation does
+    (SimulationState.cpp:286-290) and steps Barnes-Hut with the reference
+    constants; the snapshot matches the restated BarnesHut::Update run from the
+    same initial state (positions and velocities 1e-4)."""
     pu = gpu
     exe = os.path.join(PKG, "nbody_precompute")
     out = str(tmp_path / "run.nbody")
-    r = subprocess.run([exe, "-c", "-p", "3000", "--steps", "4", "--out", out, "--method", "bh"],
-                       capture_output=True, text=True, timeout=120)
+    n, steps, seed = 3000, 4, 0x5EED0006
+    r = subprocess.run([exe, "-c", "-p", str(n), "--steps", str(steps), "--out", out, "--method", "bh",
+                        "--seed", str(seed)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     got = pu.read_snapshot(out, recentre=False)
-    # the same run through the Python API
-    pos, vel, mass = pu.ic_uniform(3000, 0x5EED0006, half=500.0, mass_lo=1e20, mass_hi=1e30)
-    parts = pu.particles_from_soa(pos, np.zeros((3000, 3)), mass)
-    sim = pu.create_nbody_sim(pu.ENBodySim.BarnesHut)
-    sim.init(parts)
-    for _ in range(4):
-        sim.update(np.float32(0.02) * np.float32(1.0 / 60.0))
-    sim.close()
-    assert np.array_equal(got["position"], parts["position"])
+    # the seeder's initial state, drawn again on the device
+    with pu.NBodySim(pu.reference_config(pu.METHOD_BARNES_HUT)) as sim:
+        sim.init_seeded(pu.SEEDER_STAR_SYSTEM, n, seed)
+        p0, v0, _ = sim.read_soa()
+        init = np.zeros(n, dtype=pu.PARTICLE_DTYPE)
+        sim.read_particles(init, pu.FIELD_MASS)
+    mass = init["mass"]
+    assert mass[0] == 1e30 and np.all(p0[1:, :2] == 0)  # the star, and bodies on the z axis
+    dt = np.float32(0.02) * np.float32(1.0 / 60.0)
+    p_ref, v_ref = oracle.bh_step(p0, v0.astype(np.float64), mass, dt, steps, 2.0, [-4000.0] * 3, 8000.0)
+    assert np.array_equal(got["mass"], mass)
+    perr = rel_err_rows(got["position"], p_ref)
+    verr = rel_err_rows(got["velocity"], v_ref)
+    print(f"precompute (star system, {steps} steps): position max rel {perr.max():.2e}, velocity {verr.max():.2e}")
+    assert perr.max() < 1e-4 and verr.max() < 1e-4
     # resume from the snapshot (-f) for two more steps
     out2 = str(tmp_path / "run2.nbody")
     r = subprocess.run([exe, "-f", out, "--steps", "2", "--out", out2], capture_output=True, text=True, timeout=120)
@@ -79,4 +90,6 @@ def test_benchmark_table(gpu):
     rows = [l for l in r.stdout.splitlines() if not l.startswith("#")]
     names = [l[:16].strip() for l in rows]
     assert names == ["Brute Force CPU", "Brute Force GPU", "Barnes-Hut"]
+    # what ran: the library has no CPU simulator
+    assert [l[17:35].strip() for l in rows] == ["device direct sum", "device direct sum", "device Barnes-Hut"]
     assert all(float(l.split()[-2]) > 0 for l in rows)

@@ -193,6 +193,23 @@ def test_energy_potential_is_force_integral(oracle):
     assert dU == pytest.approx(force, rel=5e-4)
 
 
+def test_energy_momenta_closed_form(oracle):
+    """Two bodies worked by hand: P = sum m v, L = sum m x cross v, K = sum m v^2 /
+    (2 scale).  Every component differs, so a swapped index, a sign or a
+    factor (v x r instead of r x v, a missing mass) changes the answer."""
+    pos = np.array([[1, 0, 0], [0, 2, 1]], np.float32)
+    vel = np.array([[0, 3, 0], [4, 0, -2]], np.float32)
+    m = np.array([2.0, 0.5])
+    e = oracle.fp64_energy(0, pos, vel, m, 1.0, 0.1, scale=2.0)
+    assert np.array_equal(e["momentum"], [2.0, 6.0, -1.0])          # 2 (0,3,0) + 0.5 (4,0,-2)
+    assert np.array_equal(e["angular_momentum"], [-2.0, 2.0, 2.0])  # 2 (0,0,3) + 0.5 (-4,4,-8)
+    assert e["kinetic"] == pytest.approx(14.0 / 2.0, rel=1e-15)     # (2 * 9 + 0.5 * 20) / 2 / scale
+    assert e["mass"] == 2.5
+    # the law-R pair potential at r = sqrt(6): -(G m1 m2 / sqrt S)(pi/2 - atan(r / sqrt S))
+    r, S = np.sqrt(6.0), 0.1
+    assert e["potential"] == pytest.approx(-(1.0 / np.sqrt(S)) * (np.pi / 2 - np.arctan(r / np.sqrt(S))), rel=1e-12)
+
+
 def test_plummer_virial_energy(oracle, pu):
     """Plummer sphere (G = M = a = 1): E = -3 pi / 64 = -0.14726 in the large-N limit."""
     pos, vel, mass = pu.ic_plummer(4096, 0x5EED0001)
