@@ -205,6 +205,18 @@ nbody_status nbody_compute_accelerations(nbody_sim* sim, float* acc, size_t n);
  * a subsample. */
 nbody_status nbody_field(nbody_sim* sim, const float* points, size_t m, float* acc);
 
+/* nbody_field without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code between
+ * graph captures, tight serving loops, harnesses that forbid hipMalloc).  The
+ * points and their accelerations are staged in a fixed device array of
+ * NBODY_FIELD_NOALLOC_MAX_POINTS records each, which every device the library
+ * runs on holds until the process ends (2 MiB); more points give
+ * NBODY_ERR_CAPACITY.  All calls on one device share that array: calls that
+ * overlap on one device (from different threads or handles) race.  Results
+ * equal nbody_field's bit for bit. */
+#define NBODY_FIELD_NOALLOC_MAX_POINTS 65536
+nbody_status nbody_field_noalloc(nbody_sim* sim, const float* points, size_t m, float* acc);
+
 /* Number of particles of the last init. */
 nbody_status nbody_size(nbody_sim* sim, size_t* n);
 

@@ -273,6 +273,7 @@ _SIGNATURES = {
     "nbody_query": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32)]),
     "nbody_set_graph": (ctypes.c_int, [_P, ctypes.c_int32]),
     "nbody_set_theta": (ctypes.c_int, [_P, ctypes.c_double]),
+    "nbody_field_noalloc": (ctypes.c_int, [_P, _F32P, ctypes.c_size_t, _F32P]),
     "nbody_init_seeded": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_double]),
     "nbody_read": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.c_uint32]),
     "nbody_read_soa": (ctypes.c_int, [_P, _F32P, _F32P, _F32P, ctypes.c_size_t]),

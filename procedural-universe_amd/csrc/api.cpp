@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -194,6 +195,32 @@ nbody_status check_masses(const double* mass, size_t n) {
     for (size_t i = 0; i < n; ++i)
         NB_CHECK(std::isfinite(mass[i]) && mass[i] >= 0, NBODY_ERR_INVALID_ARG, "mass[%zu] = %g must be finite and >= 0",
                  i, mass[i]);
+    return NBODY_OK;
+}
+
+// Staging of nbody_field_noalloc: the points (first half) and their
+// accelerations (second half), 2 x NBODY_FIELD_NOALLOC_MAX_POINTS records,
+// 2 MiB held by every device the library runs on until the process ends.
+// In host code `field_scratch_array` names a host-side stand-in, not device
+// memory: only the address hipGetSymbolAddress returns (per device) is used.
+__device__ __align__(256) float4 field_scratch_array[2 * NBODY_FIELD_NOALLOC_MAX_POINTS];
+
+constexpr int kMaxDevices = 64;
+
+// The device address of this device's field_scratch_array, looked up once per
+// device with that device current (each device holds its own copy).
+nbody_status field_scratch(int device, float4** out) {
+    static std::mutex mu;
+    static float4* addr[kMaxDevices] = {};
+    NB_CHECK(device >= 0 && device < kMaxDevices, NBODY_ERR_UNSUPPORTED, "device %d above %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void* p = nullptr;
+        NB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(field_scratch_array)));
+        NB_CHECK(p, NBODY_ERR_DEVICE, "hipGetSymbolAddress returned no address");
+        addr[device] = static_cast<float4*>(p);
+    }
+    *out = addr[device];
     return NBODY_OK;
 }
 
@@ -637,6 +664,31 @@ nbody_status nbody_field(nbody_sim* s, const float* points, size_t m, float* acc
         acc[3 * i + 1] = h[i].y;
         acc[3 * i + 2] = h[i].z;
     }
+    return NBODY_OK;
+}
+
+nbody_status nbody_field_noalloc(nbody_sim* s, const float* points, size_t m, float* acc) {
+    clear_error();
+    NB_CHECK(s, NBODY_ERR_INVALID_ARG, "sim is null");
+    NB_CHECK(s->initialised, NBODY_ERR_NOT_INITIALISED, "nbody_field_noalloc before nbody_init");
+    NB_CHECK((points && acc) || m == 0, NBODY_ERR_INVALID_ARG, "null argument");
+    NB_CHECK(m <= NBODY_FIELD_NOALLOC_MAX_POINTS, NBODY_ERR_CAPACITY,
+             "nbody_field_noalloc takes at most %d points (%zu given); use nbody_field", NBODY_FIELD_NOALLOC_MAX_POINTS,
+             m);
+    if (m == 0) return NBODY_OK;
+    NB_HIP(hipSetDevice(s->cfg.device));
+    float4* d = nullptr;
+    NB_TRY(field_scratch(s->cfg.device, &d));
+    const int cap = NBODY_FIELD_NOALLOC_MAX_POINTS;
+    // the caller's packed points into the second half; direct_field_staged
+    // packs, evaluates and unpacks them inside the array
+    NB_HIP(hipMemcpyAsync(reinterpret_cast<float*>(d + cap), points, 3 * m * sizeof(float), hipMemcpyHostToDevice,
+                          s->stream));
+    NB_HIP(direct_field_staged(s->pos[s->cur], static_cast<int>(s->n_pad), d, cap, static_cast<int>(m), s->cfg.law,
+                               s->dscale, s->stream));
+    NB_HIP(hipMemcpyAsync(acc, reinterpret_cast<const float*>(d), 3 * m * sizeof(float), hipMemcpyDeviceToHost,
+                          s->stream));
+    NB_HIP(hipStreamSynchronize(s->stream));
     return NBODY_OK;
 }
 

@@ -120,6 +120,23 @@ __global__ __launch_bounds__(256) void field_kernel(const float4* __restrict__ s
     if (i < m) out[i] = make_float4(sx[0] * sc.out, sy[0] * sc.out, sz[0] * sc.out, 0.f);
 }
 
+// (x, y, z) floats -> float4 targets.
+__global__ __launch_bounds__(256) void pack_points_kernel(const float* __restrict__ in, int m, float4* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < m) out[i] = make_float4(in[3 * i], in[3 * i + 1], in[3 * i + 2], 0.f);
+}
+
+// float4 accelerations -> (ax, ay, az) floats.
+__global__ __launch_bounds__(256) void unpack_points_kernel(const float4* __restrict__ in, int m, float* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < m) {
+        const float4 a = in[i];
+        out[3 * i] = a.x;
+        out[3 * i + 1] = a.y;
+        out[3 * i + 2] = a.z;
+    }
+}
+
 using ForceFn = void (*)(DirectLaunch, int);
 
 template <int LAW, int K, int BLOCK>
@@ -219,6 +236,22 @@ hipError_t direct_field(const float4* src, int n_src_pad, const float4* tgt, int
         hipLaunchKernelGGL(field_kernel<NBODY_LAW_R>, grid, dim3(256), 0, stream, src, n_src_pad, tgt, m, sc, out);
     else
         hipLaunchKernelGGL(field_kernel<NBODY_LAW_P>, grid, dim3(256), 0, stream, src, n_src_pad, tgt, m, sc, out);
+    return hipGetLastError();
+}
+
+hipError_t direct_field_staged(const float4* src, int n_src_pad, float4* staging, int cap, int m, int law,
+                               const DirectScale& sc, hipStream_t stream) {
+    if (m <= 0) return hipSuccess;
+    if (m > cap) return hipErrorInvalidValue;
+    const dim3 grid((m + 255) / 256);
+    float4* half0 = staging;
+    float4* half1 = staging + cap;
+    hipLaunchKernelGGL(pack_points_kernel, grid, dim3(256), 0, stream, reinterpret_cast<const float*>(half1), m, half0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = direct_field(src, n_src_pad, half0, m, law, sc, half1, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(unpack_points_kernel, grid, dim3(256), 0, stream, half1, m, reinterpret_cast<float*>(half0));
     return hipGetLastError();
 }
 

@@ -79,4 +79,13 @@ hipError_t direct_step(const DirectLaunch& L, const DirectShape& s, bool kick, h
 hipError_t direct_field(const float4* src, int n_src_pad, const float4* tgt, int m, int law, const DirectScale& sc,
                         float4* out, hipStream_t stream);
 
+// The same in a caller's staging array of 2 * cap float4 records, without any
+// allocation: on entry the second half holds the 3m packed floats of the
+// points; the first half receives them as float4 targets, the second half
+// their accelerations, and on return the first half holds the 3m packed floats
+// of the accelerations (m <= cap).  Three stream-ordered kernels, each reading
+// one half and writing the other.
+hipError_t direct_field_staged(const float4* src, int n_src_pad, float4* staging, int cap, int m, int law,
+                               const DirectScale& sc, hipStream_t stream);
+
 }  // namespace nbody
