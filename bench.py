@@ -175,6 +175,54 @@ def bench_batch(pu, args):
     }
 
 
+def cpu_path_config(pu, oracle, cores, n=4096, steps=100, eps=0.01, dt=1e-3, seed=0x5EED0001):
+    """The reference's CPU-path configuration timed on both sides: N = 4096
+    equal-mass Plummer sphere (G = M = a = 1), law R with S = eps^2, FP32
+    state, 100 kick-drift steps.  The GPU runs the library's direct sum; the
+    CPU runs the restated BruteForceCPU::Update (FP64 velocities and sums) on
+    `cores` threads over the same input.  Reports both rates, the energy drift
+    of each and how far apart their final positions are."""
+    soft = eps * eps
+    pos, vel, mass = pu.ic_plummer(n, seed)
+    with pu.NBodySim(pu.default_config(law=pu.LAW_R, softening=soft)) as sim:
+        sim.init_soa(pos, vel, mass)
+        a0 = sim.accelerations()
+        e0 = sim.diagnostics()["total"]
+        sim.step(dt, 1)  # warm-up launch, then the state is reset
+        sim.init_soa(pos, vel, mass)
+        sim.synchronize()
+        t = time.perf_counter()
+        sim.step(dt, steps)
+        sim.synchronize()
+        t_gpu = time.perf_counter() - t
+        p_gpu, v_gpu, _ = sim.read_soa()
+        e1 = sim.diagnostics()["total"]
+    t = time.perf_counter()
+    p_cpu, v_cpu = oracle.ref_step(pos, vel.astype(np.float64), mass, dt, steps, G=1.0, S=soft, scale=1.0,
+                                   workers=cores)
+    t_cpu = time.perf_counter() - t
+    ce0 = oracle.fp64_energy(0, pos, vel, mass, 1.0, soft)["total"]
+    ce1 = oracle.fp64_energy(0, p_cpu, v_cpu.astype(np.float32), mass, 1.0, soft)["total"]
+    a64 = oracle.fp64_accel(0, pos, mass, 1.0, soft, workers=cores)
+    aerr = np.linalg.norm(a0 - a64, axis=1) / np.linalg.norm(a64, axis=1)
+    perr = np.linalg.norm(p_gpu - p_cpu, axis=1) / np.linalg.norm(p_cpu, axis=1)
+    inter = float(n) * n * steps
+    return {
+        "workload": f"N={n} Plummer sphere (G=M=a=1), law R, eps={eps}, dt={dt}, {steps} steps, FP32 state",
+        "gpu_interactions_per_s": inter / t_gpu,
+        "gpu_seconds": t_gpu,
+        "cpu_interactions_per_s": inter / t_cpu,
+        "cpu_seconds": t_cpu,
+        "cpu_cores": cores,
+        "cpu_kind": "port: restated BruteForceCPU::Update (FP32 geometry, FP64 velocities and sums)",
+        "speedup": t_cpu / t_gpu,
+        "accel_vs_fp64": {"rms_rel": float(np.sqrt(np.mean(aerr ** 2))), "max_rel": float(aerr.max())},
+        "final_position_gpu_vs_cpu_max_rel": float(perr.max()),
+        "energy_drift_gpu": float((e1 - e0) / abs(e0)),
+        "energy_drift_cpu": float((ce1 - ce0) / abs(ce0)),
+    }
+
+
 def batch_step_latency(pu, steps=50, seed=0x5EED0004, eps=0.01, dt=1e-3, count=512):
     """Per-step latency of the batched workload: 512 galaxies (N log-uniform in
     [1024, 65536], Plummer or exponential disc), autotuned, then `steps` single
@@ -316,6 +364,27 @@ def bench_bh(pu, args):
         }
     sim.close()
     return result
+
+
+def bh_summary(pu):
+    """The Barnes-Hut configuration in short (N = 4,194,304 two-disc encounter,
+    theta 0.5, eps 0.05, 5 timed steps, no CPU leg), for the default line."""
+    a = argparse.Namespace(n=1 << 22, seed=0x5EED0003, theta=0.5, eps=0.05, dt=5e-4, steps=5, warmup=1,
+                           subsample=65536, no_cpu_baseline=True)
+    r = bench_bh(pu, a)
+    c = r["config"]
+    return {
+        "workload": c["workload"],
+        "interactions_per_s": r["value"],
+        "ms_per_step": r["ms_per_step"],
+        "tree_ms_per_step": c["tree_ms_per_step"],
+        "traversal_ms_per_step": c["traversal_ms_per_step"],
+        "interactions_per_particle": c["interactions_per_particle"],
+        "lane_efficiency": c["walk"]["lane_efficiency"],
+        "traversal_frac_fp32_peak": r["roofline"]["frac"],
+        "accuracy_vs_direct": r["accuracy_vs_direct"],
+        "stack_overflow": c["stack_overflow"],
+    }
 
 
 def bench_serve(pu, args):
@@ -515,6 +584,7 @@ def main():
     ap.add_argument("--galaxies", type=int, default=512)
     ap.add_argument("--baseline-steps", type=int, default=10)
     ap.add_argument("--no-batched", action="store_true", help="direct: skip the batched step-latency leg")
+    ap.add_argument("--no-bh", action="store_true", help="direct: skip the Barnes-Hut summary leg")
     args = ap.parse_args()
 
     if args.workload == "copy":
@@ -664,6 +734,8 @@ def main():
     if not args.no_batched and n == 1 << 20:
         # the metric's second half: batched per-step latency (512 galaxies)
         result["batched"] = batch_step_latency(pu)
+    if not args.no_bh and n == 1 << 20:
+        result["barnes_hut"] = bh_summary(pu)
 
     if not args.no_cpu_baseline:
         # CPU-baseline leg: the CPU restatement is the checker of the first
@@ -689,6 +761,8 @@ def main():
         t = time.perf_counter()
         oracle.ref_forces(pos, mass, G=1.0, S=soft, rows=crow, workers=cores)
         ct = time.perf_counter() - t
+        if n == 1 << 20:
+            result["cpu_path_config"] = cpu_path_config(pu, oracle, cores)
         result["cpu_baseline"] = {
             "value": nrows * float(n) / ct,
             "unit": "interactions/s",

@@ -217,6 +217,15 @@ nbody_status nbody_field(nbody_sim* sim, const float* points, size_t m, float* a
 #define NBODY_FIELD_NOALLOC_MAX_POINTS 65536
 nbody_status nbody_field_noalloc(nbody_sim* sim, const float* points, size_t m, float* acc);
 
+/* The scaled units the direct sum computes a system in: out4 = {1/lambda,
+ * mass factor (law R: c, with h_j = c / (G m_j); law P: 1/c), softening in
+ * scaled units, c / lambda^2 (the factor that turns scaled sums back into
+ * accelerations)}, all powers of two but the softening, for a system whose
+ * positions span `extent` (the largest coordinate range) and whose largest
+ * G m is max_gm.  Computed on the host; needs no device.  For checking the
+ * pair arithmetic's range. */
+nbody_status nbody_direct_scale(int32_t law, double softening, double extent, double max_gm, float* out4);
+
 /* Number of particles of the last init. */
 nbody_status nbody_size(nbody_sim* sim, size_t* n);
 

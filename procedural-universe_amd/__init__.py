@@ -274,6 +274,7 @@ _SIGNATURES = {
     "nbody_set_graph": (ctypes.c_int, [_P, ctypes.c_int32]),
     "nbody_set_theta": (ctypes.c_int, [_P, ctypes.c_double]),
     "nbody_field_noalloc": (ctypes.c_int, [_P, _F32P, ctypes.c_size_t, _F32P]),
+    "nbody_direct_scale": (ctypes.c_int, [ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double, _F32P]),
     "nbody_init_seeded": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_double]),
     "nbody_read": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.c_uint32]),
     "nbody_read_soa": (ctypes.c_int, [_P, _F32P, _F32P, _F32P, ctypes.c_size_t]),
@@ -392,6 +393,15 @@ def _f32(a):
 
 def _f64(a):
     return a.ctypes.data_as(_F64P)
+
+
+def direct_scale(law: int, softening: float, extent: float, max_gm: float) -> dict:
+    """The scaled units the direct sum would compute such a system in
+    (nbody_direct_scale; host only)."""
+    out = np.zeros(4, np.float32)
+    _check("nbody_direct_scale", load_library().nbody_direct_scale(int(law), float(softening), float(extent),
+                                                                   float(max_gm), _f32(out)))
+    return {"inv_len": float(out[0]), "mass_k": float(out[1]), "soft": float(out[2]), "out": float(out[3])}
 
 
 def device_count() -> int:

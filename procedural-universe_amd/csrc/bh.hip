@@ -111,8 +111,9 @@ constexpr int kBucketFlag = 1 << 30;
 
 // The eight children of an octree cell, everything the traversal needs to
 // decide about them: one 192-byte record read with three 64-byte scalar loads
-// when the cell is opened.  A child cell holds its centre of mass, G M and
-// opening threshold (below); a particle its position and G m; codes are
+// when the cell is opened.  A child cell holds its centre of mass, the pair
+// arithmetic's mass factor of G M (core::mass_factor) and its opening
+// threshold (below); a particle its position and the factor of G m; codes are
 // kEmpty, enc_leaf(sorted position), or the child cell's node (| kBucketFlag).
 struct alignas(64) ChildBlock {
     float4 src[8];
@@ -618,8 +619,10 @@ struct BlockArgs {
     const float4* pos_sorted;
     int m;
     ChildBlock* blocks;
+    DirectScale sc;    // the pair arithmetic's scaled units (mass factors)
 };
 
+template <int LAW>
 __global__ __launch_bounds__(256) void block_kernel(BlockArgs A) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= A.m) return;
@@ -716,6 +719,8 @@ __global__ __launch_bounds__(256) void block_kernel(BlockArgs A) {
             const int c = code[p];
             const float4* at = c < 0 ? A.pos_sorted + dec_leaf(c) : A.node_com + c;
             src[p] = *at;
+            // the list's mass factor in place of G m, once per step
+            src[p].w = core::mass_factor<LAW>(src[p].w, A.sc);
             if (c >= 0) {
                 const int lv = info_level(word[p]);
                 th[p] = A.thr[lv];
@@ -1023,8 +1028,7 @@ struct TravArgs {
     unsigned* overflow;
     int n;
     int kick;
-    float soft;
-    float tiny;
+    DirectScale sc;  // the pair arithmetic's scaled units (the direct sum's)
     float dt;
     float dts;
 };
@@ -1045,12 +1049,15 @@ using core::kahan_add;
 // moves only the address and the lane mask.
 constexpr int kListCap = 96;
 
-// G*m where the lane's bit of the entry mask is set, else 0 (the bit
+// x where the lane's bit of the entry mask is set, else 0 (the bit
 // sign-extended to a full-width and-mask); w is the mask word holding the
 // lane's bit (lanes 0-31 read the low word, lanes 32-63 the high one).
-__device__ __forceinline__ float lane_gm(float gm, unsigned w, int lane) {
-    return __int_as_float(__float_as_int(gm) & __builtin_amdgcn_sbfe(static_cast<int>(w), lane & 31, 1));
+__device__ __forceinline__ float lane_masked(float x, unsigned w, int lane) {
+    return __int_as_float(__float_as_int(x) & __builtin_amdgcn_sbfe(static_cast<int>(w), lane & 31, 1));
 }
+
+// v_writelane_b32 through the compiler's own intrinsic is declared above the
+// namespace (nb_writelane).
 
 // (lo, hi) = x in lane l only (register writes; the other lanes keep lo and
 // hi).  l and x are uniform over the wave.
@@ -1059,27 +1066,67 @@ __device__ __forceinline__ void write_lane64(unsigned& lo, unsigned& hi, unsigne
     hi = static_cast<unsigned>(nb_writelane(static_cast<int>(x >> 32), l, static_cast<int>(hi)));
 }
 
-// A list entry is the source (x, y, z, G m) in one array and the lanes it
-// applies to in another.
+// A list entry: the source (x, y, z, mass factor) in the direct sum's scaled
+// units, as the direct sum stages its tiles (core::stage_factored):
+// (x', y', z', h) in one array and k in another, the lanes it applies to in a
+// third.  Built when the entry is written, by the lane that holds the source.
 template <int LAW>
-__device__ __forceinline__ void flush_list(const float4* lsrc, const uint2* lmask, int cnt, int lane, float px,
-                                           float py, float pz, float soft, float tiny, float& sx, float& sy,
-                                           float& sz, float& cx, float& cy, float& cz) {
+__device__ __forceinline__ void write_entry(float4* lsrc, float* lk, uint2* lmask, int at, const float4 src,
+                                            unsigned long long mask, const DirectScale& sc) {
+    float4 p;
+    float k;
+    core::stage_factored<LAW>(src, sc, p, k);
+    lsrc[at] = p;
+    lk[at] = k;
+    lmask[at] = make_uint2(static_cast<unsigned>(mask), static_cast<unsigned>(mask >> 32));
+}
+
+// The list evaluated by every lane, four entries at a time with grouped rsq
+// (the direct sum's pair arithmetic, direct_core.hpp), each term zeroed for
+// lanes outside its entry; (px, py, pz) is the lane's particle in scaled
+// units.  The sums (scaled units) go to the lane's compensated accumulator.
+template <int LAW>
+__device__ __forceinline__ void flush_list(const float4* lsrc, const float* lk, const uint2* lmask, int cnt, int lane,
+                                           float px, float py, float pz, float soft, float& sx, float& sy, float& sz,
+                                           float& cx, float& cy, float& cz) {
+    const float tiny = core::in_vgpr(__FLT_MIN__);
     float ax = 0.f, ay = 0.f, az = 0.f;
     // the lane's word of every entry's mask
     const unsigned* words = reinterpret_cast<const unsigned*>(lmask) + (lane >> 5);
     for (int e = 0; e < cnt; e += 4) {
         float4 src[4];
         float dx[4], dy[4], dz[4], q[4];
+        const float4 k4 = *reinterpret_cast<const float4*>(lk + e);
+        const float kg[4] = {k4.x, k4.y, k4.z, k4.w};
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             src[g] = lsrc[e + g];
-            src[g].w = lane_gm(src[g].w, words[(e + g) * 2], lane);
-            q[g] = core::pair_arg<LAW>(src[g], px, py, pz, soft, tiny, dx[g], dy[g], dz[g]);
+            dx[g] = src[g].x - px;
+            dy[g] = src[g].y - py;
+            dz[g] = src[g].z - pz;
+            if constexpr (LAW == NBODY_LAW_R) {
+                const float r2 = __builtin_fmaf(dz[g], dz[g], __builtin_fmaf(dy[g], dy[g], dx[g] * dx[g]));
+                const float t = __builtin_fmaf(r2, src[g].w, kg[g]);
+                const float u = r2 * t;
+                q[g] = __builtin_fmaf(u, t, tiny);
+            } else {
+                q[g] = __builtin_fmaf(dz[g], dz[g], __builtin_fmaf(dy[g], dy[g], __builtin_fmaf(dx[g], dx[g], soft)));
+            }
         }
         core::rsq4(q[0], q[1], q[2], q[3]);
 #pragma unroll
-        for (int g = 0; g < 4; ++g) core::pair_acc<LAW>(src[g].w, q[g], dx[g], dy[g], dz[g], ax, ay, az);
+        for (int g = 0; g < 4; ++g) {
+            float f;
+            if constexpr (LAW == NBODY_LAW_R) {
+                f = q[g];
+            } else {
+                f = (src[g].w * q[g]) * (q[g] * q[g]);
+            }
+            f = lane_masked(f, words[(e + g) * 2], lane);
+            ax = __builtin_fmaf(dx[g], f, ax);
+            ay = __builtin_fmaf(dy[g], f, ay);
+            az = __builtin_fmaf(dz[g], f, az);
+        }
     }
     kahan_add(sx, cx, ax);
     kahan_add(sy, cy, ay);
@@ -1110,6 +1157,7 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
     __shared__ int stk_node[kTravWaves][kStackCap];
     __shared__ uint2 stk_mask[kTravWaves][kStackCap];
     __shared__ float4 lsrc[kTravWaves][kListCap];
+    __shared__ __align__(16) float lk[kTravWaves][kListCap];
     __shared__ uint2 lmask[kTravWaves][kListCap];
     __shared__ unsigned long long red[3][kTravWaves];
     const int tid = threadIdx.x, lane = tid & 63;
@@ -1119,8 +1167,9 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
     const int wave_base = blk * kTravThreads + wave * 64;
     const bool valid = t < A.n;
     const float4 me = pos_sorted[valid ? t : A.n - 1];
-    const float soft = core::in_vgpr(A.soft);
-    const float tiny = core::in_vgpr(A.tiny);
+    // the particle in the pair arithmetic's scaled units (the decisions use me)
+    const float mx = me.x * A.sc.inv_len, my = me.y * A.sc.inv_len, mz = me.z * A.sc.inv_len;
+    const float soft = core::in_vgpr(A.sc.soft);
     float sx = 0.f, sy = 0.f, sz = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     // wave-uniform counters and list length
     unsigned long long count = 0, entries = 0, opened = 0;
@@ -1136,26 +1185,25 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
     // lanes beyond the batch's children are not used
     unsigned ge_lo = 0u, ge_hi = 0u;
 
-    // Append a source held in scalar registers (written by lane 0).
+    // Append a source (x, y, z, G m) held in scalar registers (written by lane 0).
     auto append = [&](const float4 src, unsigned long long mask) {
-        if (lane == 0) {
-            lsrc[wave][cnt] = src;
-            lmask[wave][cnt] = make_uint2(static_cast<unsigned>(mask), static_cast<unsigned>(mask >> 32));
-        }
+        if (lane == 0)
+            write_entry<LAW>(lsrc[wave], lk[wave], lmask[wave], cnt,
+                             make_float4(src.x, src.y, src.z, core::mass_factor<LAW>(src.w, A.sc)), mask, A.sc);
         count += static_cast<unsigned long long>(__popcll(mask));
         ++cnt;
     };
     auto flush = [&]() {
         entries += static_cast<unsigned long long>(cnt);
         const int pad = (4 - (cnt & 3)) & 3;
-        if (lane < pad) {
+        if (lane < pad) {  // masked for every lane: the terms are zeroed whatever they are
             lsrc[wave][cnt + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+            lk[wave][cnt + lane] = 0.f;
             lmask[wave][cnt + lane] = make_uint2(0u, 0u);
         }
         const int total = __builtin_amdgcn_readfirstlane(cnt + pad);
         __builtin_amdgcn_wave_barrier();
-        flush_list<LAW>(lsrc[wave], lmask[wave], total, lane, me.x, me.y, me.z, soft, tiny, sx, sy, sz, cx, cy,
-                        cz);
+        flush_list<LAW>(lsrc[wave], lk[wave], lmask[wave], total, lane, mx, my, mz, soft, sx, sy, sz, cx, cy, cz);
         __builtin_amdgcn_wave_barrier();
         cnt = 0;
         lane_total += lane_count;
@@ -1316,8 +1364,7 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         if (wl) {
             const int at = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
                 static_cast<unsigned>(has_l >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(has_l), cnt)));
-            lsrc[wave][at] = vsrc;
-            lmask[wave][at] = make_uint2(static_cast<unsigned>(am), static_cast<unsigned>(am >> 32));
+            write_entry<LAW>(lsrc[wave], lk[wave], lmask[wave], at, vsrc, am, A.sc);
             lane_count += __popcll(am);
         }
         cnt += __popcll(has_l);
@@ -1359,13 +1406,15 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
 
     if (valid) {
         const unsigned p = A.order[t];
-        A.acc[p] = make_float4(sx, sy, sz, 0.f);
+        // back from scaled units (a power of two: exact)
+        const float ax = sx * A.sc.out, ay = sy * A.sc.out, az = sz * A.sc.out;
+        A.acc[p] = make_float4(ax, ay, az, 0.f);
         if (A.kick) {
             float4 v = A.vel[p];
             float4 x = A.pos_in[p];
-            v.x = __builtin_fmaf(sx, A.dt, v.x);
-            v.y = __builtin_fmaf(sy, A.dt, v.y);
-            v.z = __builtin_fmaf(sz, A.dt, v.z);
+            v.x = __builtin_fmaf(ax, A.dt, v.x);
+            v.y = __builtin_fmaf(ay, A.dt, v.y);
+            v.z = __builtin_fmaf(az, A.dt, v.z);
             x.x = __builtin_fmaf(v.x, A.dts, x.x);
             x.y = __builtin_fmaf(v.y, A.dts, x.y);
             x.z = __builtin_fmaf(v.z, A.dts, x.z);
@@ -1650,9 +1699,12 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     }
     hipLaunchKernelGGL(cell_com_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->prefix, n, b->first, b->last,
                        b->info, m, s->cfg.G, b->com);
-    BlockArgs L{b->child, b->info, b->com, b->thr, b->pos_sorted, m, b->blocks};
+    BlockArgs L{b->child, b->info, b->com, b->thr, b->pos_sorted, m, b->blocks, s->dscale};
     b->built = true;
-    hipLaunchKernelGGL(block_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, L);
+    if (s->cfg.law == NBODY_LAW_R)
+        hipLaunchKernelGGL(block_kernel<NBODY_LAW_R>, dim3(blocks_for(m, 256)), dim3(256), 0, st, L);
+    else
+        hipLaunchKernelGGL(block_kernel<NBODY_LAW_P>, dim3(blocks_for(m, 256)), dim3(256), 0, st, L);
     NB_HIP(hipGetLastError());
     return NBODY_OK;
 }
@@ -1705,8 +1757,7 @@ nbody_status bh_step(nbody_sim* s, float dt, bool kick, hipEvent_t ev_b, hipEven
     A.overflow = b->counters + 1;
     A.n = n;
     A.kick = kick ? 1 : 0;
-    A.soft = static_cast<float>(s->cfg.softening);
-    A.tiny = direct_tiny_for(s->max_gm);
+    A.sc = s->dscale;
     A.dt = dt;
     A.dts = static_cast<float>(static_cast<double>(dt) / s->cfg.scale);
     if (ev_b) NB_HIP(hipEventRecord(ev_b, st));

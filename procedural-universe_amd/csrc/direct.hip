@@ -24,6 +24,7 @@
 #include <cmath>
 
 #include "../../include/nbody.h"
+#include "common.hpp"
 #include "direct.hpp"
 #include "direct_core.hpp"
 
@@ -179,14 +180,19 @@ bool direct_shape_valid(const DirectShape& s) {
 
 DirectScale direct_scale_for(int law, double softening, double extent, double max_gm) {
     DirectScale sc{};
-    // lambda: the power of two that puts the larger of the extent and the
-    // softening length at 32 to 64 scaled units
-    const double len = std::max(extent, std::sqrt(std::max(softening, 0.0)));
+    // lambda: the power of two nearest the geometric mean of the shortest and
+    // longest distances that matter, so that their sixth powers (law R's rsq
+    // argument grows as r^6) sit symmetrically inside FP32's range.  The
+    // longest is the diagonal of the extent; the shortest is the softening
+    // length, or the spacing of FP32 positions across the extent (2^-24 of it)
+    // when that is larger.
+    const double r_hi = std::sqrt(3.0) * std::max(extent, 0.0);
+    const double r_lo = std::max(std::sqrt(std::max(softening, 0.0)), std::ldexp(std::max(extent, 0.0), -24));
     int e_len = 0;
-    if (len > 0 && std::isfinite(len)) {
-        (void)std::frexp(len, &e_len);  // len in [2^(e-1), 2^e)
-        e_len -= 6;
-    }
+    if (r_hi > 0 && r_lo > 0 && std::isfinite(r_hi) && std::isfinite(r_lo))
+        e_len = static_cast<int>(std::lround(0.5 * (std::log2(r_hi) + std::log2(r_lo))));
+    else if (r_lo > 0 && std::isfinite(r_lo))
+        e_len = static_cast<int>(std::lround(std::log2(r_lo)));
     e_len = std::max(-120, std::min(120, e_len));
     // c: the power of two at or below the largest G m
     int e_mass = 0;
@@ -202,12 +208,6 @@ DirectScale direct_scale_for(int law, double softening, double extent, double ma
     sc.soft = static_cast<float>(std::ldexp(softening, -2 * e_len));
     sc.out = std::ldexp(1.0f, e_mass - 2 * e_len);
     return sc;
-}
-
-float direct_tiny_for(double max_gm) {
-    const double lim = 2.0 * (max_gm > 0 ? max_gm : 0.0) / static_cast<double>(FLT_MAX);
-    const double t = 4.0 * lim * lim;
-    return static_cast<float>(t > static_cast<double>(FLT_MIN) ? t : static_cast<double>(FLT_MIN));
 }
 
 int direct_effective_split(const DirectShape& s, int n_pad) {
@@ -287,3 +287,16 @@ hipError_t direct_step(const DirectLaunch& L, const DirectShape& s, bool kick, h
 }
 
 }  // namespace nbody
+
+extern "C" nbody_status nbody_direct_scale(int32_t law, double softening, double extent, double max_gm,
+                                           float* out4) {
+    nbody::clear_error();
+    NB_CHECK(out4, NBODY_ERR_INVALID_ARG, "out is null");
+    NB_CHECK(law == NBODY_LAW_R || law == NBODY_LAW_P, NBODY_ERR_INVALID_ARG, "unknown law %d", law);
+    const nbody::DirectScale sc = nbody::direct_scale_for(law, softening, extent, max_gm);
+    out4[0] = sc.inv_len;
+    out4[1] = sc.mass_k;
+    out4[2] = sc.soft;
+    out4[3] = sc.out;
+    return NBODY_OK;
+}

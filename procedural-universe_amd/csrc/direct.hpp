@@ -23,12 +23,15 @@ struct DirectShape {
 // Powers of two in which a system's pair arithmetic runs (exact rescalings):
 // positions are multiplied by inv_len = 1 / lambda, masses enter as
 // c / (G m) (law R) or G m / c (law P), and the sums come back multiplied by
-// out = c / lambda^2.  lambda puts the larger of the system's extent and its
-// softening length (sqrt S or eps) at 32 to 64 scaled units, c is the power of
-// two at or below the largest G m.  Law R's rsq argument r'^2 (r'^2 + S')^2
-// (c / G m_j)^2 then stays inside FP32's normal range for every pair of the
-// system (and for systems that later grow up to ~1e4 times) unless a source
-// is lighter than about 1e-11 of the heaviest, whose pull is then dropped.
+// out = c / lambda^2.  lambda is the power of two nearest the geometric mean
+// of the longest distance in the system (its extent's diagonal) and the
+// shortest that matters (the softening length, or FP32's position spacing
+// across the extent, 2^-24 of it, if larger); c is the power of two at or
+// below the largest G m.  Law R's rsq argument r'^2 (r'^2 + S')^2 (c / G m_j)^2
+// grows as r^6, so centring the distances' logarithms keeps it inside FP32's
+// normal range for every pair that matters, with room for the system to grow
+// or shrink some hundred times; a source lighter than about 1e-11 of the
+// heaviest loses its pull beyond that range first (it is then dropped).
 struct DirectScale {
     float inv_len;  // 1 / lambda
     float mass_k;   // law R: c; law P: 1 / c
@@ -56,9 +59,6 @@ struct DirectLaunch {
 
 bool direct_shape_valid(const DirectShape& s);
 
-// Smallest `tiny` (>= FLT_MIN) for which max_gm / sqrt(tiny) stays below
-// FLT_MAX / 2 (the Barnes-Hut list evaluation's guard).
-float direct_tiny_for(double max_gm);
 
 // Effective number of j-chunks for a shape at this size (never an empty chunk).
 int direct_effective_split(const DirectShape& s, int n_pad);

@@ -11,17 +11,18 @@
 //
 // Law R (reference, src/Sim/Physics.hpp:25-35), per source j staged in LDS as
 // (x', y', z', h) and k with h = c / (G m_j), k = S' h (S' = S / lambda^2):
-//     r2 = |d'|^2 + tiny0,  t = fma(r2, h, k) = (r2 + S') c / (G m_j),
+//     r2 = |d'|^2,  t = fma(r2, h, k) = (r2 + S') c / (G m_j),
 //     q  = r2 t^2 + tiny,   a' += d' * rsq(q)
 // so rsq(q) = (G m_j / c) / (|d'| (r2 + S')): the source's mass is folded into
 // the reciprocal square root and the pair costs 12 add/sub/mul/fma and one
 // v_rsq_f32 (the unfolded form needs a 13th multiply by G m_j).  A massless
-// source (the padding dummies) has h = k = +inf: t, q = +inf and rsq = 0
-// exactly; tiny0 = FLT_MIN keeps r2 > 0 so that no 0 * inf appears.  tiny
-// keeps a coincident pair (j == i included) finite: q >= tiny, so d' = 0
-// makes the term exactly 0, as the reference's Normalize of a zero vector
-// does; for r2 > 0 it changes q by tiny / q (nothing above q ~ 1e-31).
-// Masses must be >= 0 (the square loses a negative mass's sign).
+// source (the padding dummies) is staged at (2^100, 2^100, 2^100) with
+// h = k = +inf: its r2 overflows to +inf (never 0, so no 0 * inf appears),
+// t and q are +inf and rsq = 0 exactly, and d' * 0 = 0.  tiny keeps a
+// coincident pair (j == i included) finite: q >= tiny, so d' = 0 makes the
+// term exactly 0, as the reference's Normalize of a zero vector does; for
+// r2 > 0 it changes q by tiny / q (nothing above q ~ 1e-31).  Masses must be
+// >= 0 (the square loses a negative mass's sign).
 //
 // Law P (Plummer), per source (x', y', z', w) with w = G m_j / c:
 //     a' += d' * w * rsq(r2 + eps'^2)^3.
@@ -89,39 +90,6 @@ __device__ __forceinline__ void rsq4(float& a, float& b, float& c, float& d) {
         : "+v"(a), "+v"(b), "+v"(c), "+v"(d));
 }
 
-// Argument of the reciprocal square root for one pair: r2 (r2 + S)^2 + tiny
-// (law R) or r2 + eps^2 (law P), with the difference vector d = x_j - x_i.
-template <int LAW>
-__device__ __forceinline__ float pair_arg(const float4 pj, const float xi, const float yi, const float zi,
-                                          const float soft, const float tiny, float& dx, float& dy, float& dz) {
-    dx = pj.x - xi;
-    dy = pj.y - yi;
-    dz = pj.z - zi;
-    if constexpr (LAW == NBODY_LAW_R) {
-        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-        const float t = r2 + soft;
-        const float u = r2 * t;
-        return __builtin_fmaf(u, t, tiny);
-    } else {
-        return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, soft)));
-    }
-}
-
-// acc += d * G m_j * f(rsq): law R uses rsq directly, law P its cube.
-template <int LAW>
-__device__ __forceinline__ void pair_acc(const float gm, const float inv, const float dx, const float dy,
-                                         const float dz, float& ax, float& ay, float& az) {
-    float s;
-    if constexpr (LAW == NBODY_LAW_R) {
-        s = gm * inv;
-    } else {
-        s = (gm * inv) * (inv * inv);
-    }
-    ax = __builtin_fmaf(dx, s, ax);
-    ay = __builtin_fmaf(dy, s, ay);
-    az = __builtin_fmaf(dz, s, az);
-}
-
 // One staged source tile: positions in scaled units with the law's per-source
 // factor (h for law R, w for law P) in .w, and law R's k = S' h in a second
 // array (so that four sources' k come in one 16-byte read).
@@ -130,21 +98,42 @@ struct SrcTile {
     float k[kTile];
 };
 
+// A source's mass factor: h = c / (G m) for law R (+inf when massless), or
+// w = G m / c for law P.
+template <int LAW>
+__device__ __forceinline__ float mass_factor(const float gm, const DirectScale& sc) {
+    if constexpr (LAW == NBODY_LAW_R) {
+        return gm > 0.f ? __fdiv_rn(sc.mass_k, gm) : __builtin_inff();
+    } else {
+        return gm * sc.mass_k;
+    }
+}
+
+// The staged record of a source whose .w already holds its mass factor: law R
+// stages a massless source (h = +inf) at (2^100, 2^100, 2^100) with k = +inf.
+template <int LAW>
+__device__ __forceinline__ void stage_factored(const float4 s, const DirectScale& sc, float4& p, float& k) {
+    if constexpr (LAW == NBODY_LAW_R) {
+        const bool live = s.w != __builtin_inff();
+        const float far = 0x1p100f;
+        p.x = live ? s.x * sc.inv_len : far;
+        p.y = live ? s.y * sc.inv_len : far;
+        p.z = live ? s.z * sc.inv_len : far;
+        p.w = s.w;
+        k = live ? sc.soft * s.w : __builtin_inff();
+    } else {
+        p.x = s.x * sc.inv_len;
+        p.y = s.y * sc.inv_len;
+        p.z = s.z * sc.inv_len;
+        p.w = s.w;
+        k = 0.f;
+    }
+}
+
 // The staged record of a source (x, y, z, G m) of this system.
 template <int LAW>
 __device__ __forceinline__ void stage_source(const float4 s, const DirectScale& sc, float4& p, float& k) {
-    p.x = s.x * sc.inv_len;
-    p.y = s.y * sc.inv_len;
-    p.z = s.z * sc.inv_len;
-    if constexpr (LAW == NBODY_LAW_R) {
-        const bool live = s.w > 0.f;
-        const float h = live ? __fdiv_rn(sc.mass_k, s.w) : __builtin_inff();
-        p.w = h;
-        k = live ? sc.soft * h : __builtin_inff();
-    } else {
-        p.w = s.w * sc.mass_k;
-        k = 0.f;
-    }
+    stage_factored<LAW>(make_float4(s.x, s.y, s.z, mass_factor<LAW>(s.w, sc)), sc, p, k);
 }
 
 // Sums of the accelerations on the K targets (xi, yi, zi) of this lane, in
@@ -161,7 +150,6 @@ __device__ __forceinline__ void block_forces(SrcTile* tiles, const float4* __res
     static_assert(kTile % BLOCK == 0, "tile must be a multiple of the block");
     constexpr int PER = kTile / BLOCK;
     const int tid = threadIdx.x;
-    const float tiny0 = in_vgpr(__FLT_MIN__);
     const float tiny = in_vgpr(__FLT_MIN__);
     const float soft = in_vgpr(sc.soft);
     float cx[K], cy[K], cz[K];
@@ -208,7 +196,7 @@ __device__ __forceinline__ void block_forces(SrcTile* tiles, const float4* __res
                     dy[g] = pj[g].y - yi[k];
                     dz[g] = pj[g].z - zi[k];
                     if constexpr (LAW == NBODY_LAW_R) {
-                        const float r2 = __builtin_fmaf(dz[g], dz[g], __builtin_fmaf(dy[g], dy[g], __builtin_fmaf(dx[g], dx[g], tiny0)));
+                        const float r2 = __builtin_fmaf(dz[g], dz[g], __builtin_fmaf(dy[g], dy[g], dx[g] * dx[g]));
                         const float t = __builtin_fmaf(r2, pj[g].w, kg[g]);
                         const float u = r2 * t;
                         q[g] = __builtin_fmaf(u, t, tiny);

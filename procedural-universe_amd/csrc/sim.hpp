@@ -37,7 +37,7 @@ struct nbody_sim {
     double* ic_scratch = nullptr;         // device initial conditions: recentring sums
     size_t diag_elems = 0;
     std::vector<double> mass_host;        // FP64 masses (FORCES read-back = m * a)
-    double max_gm = 0.0;                  // largest |G*m| (sizes the law-R guard)
+    double max_gm = 0.0;                  // largest |G*m| (the direct sum's mass scale)
     nbody::DirectScale dscale{1.f, 1.f, 0.f, 1.f};  // direct sum's scaled units
     bool acc_valid = false;
 

@@ -222,16 +222,18 @@ def law_r_force(p, q, mp, mq, G=REF_G, S=REF_S):
     return G * mp * mq / (r * r + S) * d / r
 
 
-def bh_step_dynamic(oracle, pos, vel, mass, dt, nsteps, theta, G=1.0, S=1e-4, scale=1.0, com_fp64=True):
+def bh_step_dynamic(oracle, pos, vel, mass, dt, nsteps, theta, G=1.0, S=1e-4, scale=1.0, com_fp64=True, root=None):
     """BarnesHut::Update (src/Sim/BarnesHut.cpp:44-96) repeated nsteps times
     with the device's dynamic root (the bounding box grown to a power-of-two
-    cube, recomputed every step) instead of the reference's fixed one: the
-    oracle's octree forces, then v += F/m dt in FP64 and x += (float)(v dt/scale)."""
+    cube, recomputed every step) instead of the reference's fixed one, or with
+    the fixed root=(lo, size): the oracle's octree forces, with FP64 cell
+    centres of mass as the device computes them unless com_fp64 is False, then
+    v += F/m dt in FP64 and x += (float)(v dt/scale)."""
     p = np.ascontiguousarray(pos, dtype=np.float32).copy()
     v = np.asarray(vel, dtype=np.float64).copy()
     m = np.asarray(mass, dtype=np.float64)
     for _ in range(int(nsteps)):
-        lo, size = oracle.dynamic_root(p)
+        lo, size = oracle.dynamic_root(p) if root is None else root
         F, _ = oracle.bh_forces(p, m, theta, lo, size, G=G, S=S, com_fp64=com_fp64)
         v += F / m[:, None] * np.float64(np.float32(dt))
         p = (p.astype(np.float32) + (v * np.float64(np.float32(dt)) / scale).astype(np.float32)).astype(np.float32)

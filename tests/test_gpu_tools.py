@@ -5,7 +5,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from helpers import REF_G, REF_S, ROOT, rel_err_rows
+from helpers import REF_G, REF_S, REF_SCALE, ROOT, bh_step_dynamic, rel_err_rows
 
 pytestmark = pytest.mark.gpu
 
@@ -50,7 +50,10 @@ def test_precompute_driver_writes_snapshot(gpu, oracle, tmp_path):
     """`-c` seeds with the StarSystem seeder as RunSimulation does
     (SimulationState.cpp:286-290) and steps Barnes-Hut with the reference
     constants; the snapshot matches the restated BarnesHut::Update run from the
-    same initial state (positions and velocities 1e-4)."""
+    same initial state on the same fixed root with FP64 cell centres of mass,
+    the device tree's arithmetic (positions and velocities 1e-4; with the
+    reference's FP32 centres some opening decisions at theta 2 flip, and a
+    flipped cell moves a body's velocity by up to ~3e-4)."""
     pu = gpu
     exe = os.path.join(PKG, "nbody_precompute")
     out = str(tmp_path / "run.nbody")
@@ -68,7 +71,8 @@ def test_precompute_driver_writes_snapshot(gpu, oracle, tmp_path):
     mass = init["mass"]
     assert mass[0] == 1e30 and np.all(p0[1:, :2] == 0)  # the star, and bodies on the z axis
     dt = np.float32(0.02) * np.float32(1.0 / 60.0)
-    p_ref, v_ref = oracle.bh_step(p0, v0.astype(np.float64), mass, dt, steps, 2.0, [-4000.0] * 3, 8000.0)
+    p_ref, v_ref = bh_step_dynamic(oracle, p0, v0, mass, dt, steps, 2.0, G=REF_G, S=REF_S, scale=REF_SCALE,
+                                   root=([-4000.0] * 3, 8000.0))
     assert np.array_equal(got["mass"], mass)
     perr = rel_err_rows(got["position"], p_ref)
     verr = rel_err_rows(got["velocity"], v_ref)
@@ -93,3 +97,28 @@ def test_benchmark_table(gpu):
     # what ran: the library has no CPU simulator
     assert [l[17:35].strip() for l in rows] == ["device direct sum", "device direct sum", "device Barnes-Hut"]
     assert all(float(l.split()[-2]) > 0 for l in rows)
+
+
+def test_adapter_thousand_frames_against_oracle(gpu, oracle):
+    """Update() through the adapter 1,000 times (reference constants, a
+    RandomSeeder-like set): the device keeps FP32 velocities where the
+    reference keeps FP64 ones, and the caller's FP64 Velocity fields receive
+    the FP32 values every frame (the device state never reads them back).
+    After 1,000 frames the positions stay within 1e-4 of the restated
+    BruteForceCPU::Update and the velocities within 1e-4."""
+    pu = gpu
+    n, frames = 512, 1000
+    pos, _, mass = pu.ic_uniform(n, 0x5EED0400, half=500.0, mass_lo=1e20, mass_hi=1e30)
+    vel = (pos / np.linalg.norm(pos, axis=1, keepdims=True) * 1e16).astype(np.float32)
+    parts = pu.particles_from_soa(pos, vel.astype(np.float64), mass)
+    dt = np.float32(0.02 / 60)
+    sim = pu.create_nbody_sim(pu.ENBodySim.BruteForceCPU)
+    sim.init(parts)
+    for _ in range(frames):
+        sim.update(dt)
+    sim.close()
+    p_ref, v_ref = oracle.ref_step(pos, vel.astype(np.float64), mass, dt, frames)
+    perr = rel_err_rows(parts["position"], p_ref)
+    verr = rel_err_rows(parts["velocity"], v_ref)
+    print(f"1000 frames: position max rel {perr.max():.2e}, velocity max rel {verr.max():.2e}")
+    assert perr.max() < 1e-4 and verr.max() < 1e-4
