@@ -33,13 +33,19 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector peak (AMD public spec)
 # HBM-side bytes per launch of the fused direct kernel at N = 2^20 with the
 # default shape, from rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by a 256 MiB
 # copy (profiles/direct_force_n1m_law_r_traffic.md).
-DIRECT_1M_TRAFFIC_BYTES = 2.1854e9
+DIRECT_1M_TRAFFIC_BYTES = 2.1835e9
+# The same launch's SQ counters (profiles/direct_force_n1m_law_r_counters.md):
+# VALU instructions per pair, VALU instructions issued per SIMD per cycle and
+# rocprof's VALUBusy, at the 2.265 GHz the launch ran at.
+DIRECT_1M_VALU = {"valu_instructions_per_pair": 13.30, "rsq_per_pair": 1.004, "valu_issue_per_simd_per_clock": 0.326,
+                  "valu_lane_ops_per_clock_per_cu": 83.5, "valu_busy_pct": 140, "lds_wait_pct_of_wave_time": 1.0,
+                  "clock_ghz": 2.265, "source": "rocprofv3 --pmc, profiles/direct_force_n1m_law_r_counters.md"}
 # The same for the Barnes-Hut traversal step launch at N = 2^22, theta 0.5
 # (profiles/bh_traverse_n4m_theta05_traffic.md) and the 512-galaxy batch step
 # (profiles/batch_step_512_galaxies_traffic.md); they hold for the default
 # workloads only.
-BH_4M_TRAFFIC_BYTES = 3.5716e+09
-BATCH_512_TRAFFIC_BYTES = 1.4884e9
+BH_4M_TRAFFIC_BYTES = 2.9858e+09
+BATCH_512_TRAFFIC_BYTES = 1.6415e9
 FLOP_PER_INTERACTION = 20  # Nyland, Harris & Prins (GPU Gems 3, ch. 31) convention
 METRIC = "interactions/s & % FP32 VALU peak at N=1M; batched p50/p99 step latency"
 
@@ -719,6 +725,9 @@ def main():
             "kernel": "direct_force_kernel (fused kick-drift)",
             "kernel_ms": force_ms,
             "flop_per_interaction": FLOP_PER_INTERACTION,
+            "valu_counters": DIRECT_1M_VALU
+            if (n == 1 << 20 and timing.i_per_lane == 1 and timing.block == 256 and timing.split == 1)
+            else None,
         },
     }
     if e0 is not None:
