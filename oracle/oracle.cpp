@@ -200,6 +200,17 @@ struct Tree {
             bool any_p = false, any_P = false;
             for (int k = 0; k < 8; ++k) {
                 const int ch = nodes[id].child[k];
+                // A cube one FP32 ulp wide splits into children as wide as
+                // itself (lo + size/2 rounds up to the next float), so two
+                // particles that share such a child are never separated: the
+                // reference recurses until either a child cube comes out
+                // empty (hi == lo: both dropped) or, when none does, until
+                // its stack overflows.  The oracle drops both at the first
+                // split that makes no progress, which is the reference's
+                // outcome whenever the reference ends at all.
+                if (nodes[ch].size >= nodes[id].size && nodes[ch].bounds.contains(pp) &&
+                    nodes[ch].bounds.contains(pP))
+                    continue;
                 if (nodes[ch].bounds.contains(pp)) {
                     add(ch, p);
                     any_p = true;
@@ -546,7 +557,7 @@ void oracle_sort_order(size_t n, const uint64_t* keys, uint32_t* order) {
     std::vector<uint32_t> idx(n);
     std::iota(idx.begin(), idx.end(), 0u);
     std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
-    std::memcpy(order, idx.data(), n * sizeof(uint32_t));
+    std::copy(idx.begin(), idx.end(), order);
 }
 
 float oracle_dynamic_root(size_t n, const float* pos, float* out_lo) {

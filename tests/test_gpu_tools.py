@@ -99,26 +99,68 @@ def test_benchmark_table(gpu):
     assert all(float(l.split()[-2]) > 0 for l in rows)
 
 
-def test_adapter_thousand_frames_against_oracle(gpu, oracle):
-    """Update() through the adapter 1,000 times (reference constants, a
-    RandomSeeder-like set): the device keeps FP32 velocities where the
-    reference keeps FP64 ones, and the caller's FP64 Velocity fields receive
-    the FP32 values every frame (the device state never reads them back).
-    After 1,000 frames the positions stay within 1e-4 of the restated
-    BruteForceCPU::Update and the velocities within 1e-4."""
-    pu = gpu
-    n, frames = 512, 1000
-    pos, _, mass = pu.ic_uniform(n, 0x5EED0400, half=500.0, mass_lo=1e20, mass_hi=1e30)
-    vel = (pos / np.linalg.norm(pos, axis=1, keepdims=True) * 1e16).astype(np.float32)
+def _adapter_frames(pu, pos, vel, mass, dt, frames):
     parts = pu.particles_from_soa(pos, vel.astype(np.float64), mass)
-    dt = np.float32(0.02 / 60)
     sim = pu.create_nbody_sim(pu.ENBodySim.BruteForceCPU)
     sim.init(parts)
     for _ in range(frames):
         sim.update(dt)
     sim.close()
+    return parts["position"], parts["velocity"]
+
+
+def _oracle_fp32_velocity(oracle, pos, vel, mass, dt, frames):
+    """The restated BruteForceCPU::Update with the velocity rounded to FP32
+    after every frame: the reference's arithmetic on the device's state
+    precision."""
+    p, v = pos.copy(), vel.astype(np.float64)
+    for _ in range(frames):
+        p, v = oracle.ref_step(p, v, mass, dt, 1)
+        v = v.astype(np.float32).astype(np.float64)
+    return p, v
+
+
+@pytest.mark.parametrize("half", [2000.0, 500.0])
+def test_adapter_thousand_frames_against_oracle(gpu, oracle, half):
+    """Update() through the adapter 1,000 times (reference constants, dt =
+    0.02/60, a RandomSeeder-like set of 512 particles with masses 1e20-1e30 and
+    radial velocities of 1e16) against the restated BruteForceCPU::Update.
+    The device keeps FP32 velocities where the reference keeps FP64 ones, and
+    the caller's FP64 Velocity fields receive the FP32 values every frame.
+
+    Cube half-width 2,000: the system is not chaotic over 1,000 frames (the
+    reference started from velocities moved by one FP32 ulp stays within 2e-7
+    of itself), and the reference with its velocity rounded to FP32 after
+    every frame drifts from itself by up to ~2e-5 in position and ~2e-3 in
+    velocity (relative, per particle): that is what FP32 velocity state costs.
+    The device is held to a small multiple of that drift.
+
+    Cube half-width 500: close encounters make the system chaotic; the
+    reference itself, started one FP32 ulp away, ends up to ~5e-2 (position)
+    and ~9 (velocity) away at the few particles that pass close to another.
+    The device is held to the reference's own sensitivity: its median particle
+    agrees to 1e-5, and its largest deviations stay within a small multiple of
+    the perturbed reference's."""
+    pu = gpu
+    n, frames = 512, 1000
+    pos, _, mass = pu.ic_uniform(n, 0x5EED0400, half=half, mass_lo=1e20, mass_hi=1e30)
+    vel = (pos / np.linalg.norm(pos, axis=1, keepdims=True) * 1e16).astype(np.float32)
+    dt = np.float32(0.02 / 60)
+    p_dev, v_dev = _adapter_frames(pu, pos, vel, mass, dt, frames)
     p_ref, v_ref = oracle.ref_step(pos, vel.astype(np.float64), mass, dt, frames)
-    perr = rel_err_rows(parts["position"], p_ref)
-    verr = rel_err_rows(parts["velocity"], v_ref)
-    print(f"1000 frames: position max rel {perr.max():.2e}, velocity max rel {verr.max():.2e}")
-    assert perr.max() < 1e-4 and verr.max() < 1e-4
+    p_ulp, v_ulp = oracle.ref_step(pos, vel.astype(np.float64) * (1 + 2.0**-24), mass, dt, frames)
+    p_f32, v_f32 = _oracle_fp32_velocity(oracle, pos, vel, mass, dt, frames)
+    perr, verr = rel_err_rows(p_dev, p_ref), rel_err_rows(v_dev, v_ref)
+    pulp, vulp = rel_err_rows(p_ulp, p_ref), rel_err_rows(v_ulp, v_ref)
+    pf32, vf32 = rel_err_rows(p_f32, p_ref), rel_err_rows(v_f32, v_ref)
+    q = lambda e: "median %.1e p99 %.1e max %.1e" % (np.median(e), np.quantile(e, 0.99), e.max())
+    print(f"\nhalf {half}: device position {q(perr)} velocity {q(verr)}")
+    print(f"  reference one ulp away: position {q(pulp)} velocity {q(vulp)}")
+    print(f"  reference, FP32 velocity: position {q(pf32)} velocity {q(vf32)}")
+    if half == 2000.0:
+        assert pulp.max() < 1e-6  # not chaotic
+        assert perr.max() < max(1e-4, 5 * pf32.max())
+        assert verr.max() < max(1e-3, 5 * vf32.max())
+    else:
+        assert np.median(perr) < 1e-5 and np.median(verr) < 1e-4
+        assert perr.max() < 10 * pulp.max() and verr.max() < 10 * vulp.max()

@@ -364,3 +364,24 @@ def test_bh_opening_test_at_the_boundary(oracle, k):
     assert np.linalg.norm(exact - mono) > 1e-2 * np.linalg.norm(exact)
     want = mono if k >= 1 else exact
     assert np.linalg.norm(F[t] - want) < 1e-5 * np.linalg.norm(want)
+
+
+@pytest.mark.parametrize("c", [(-9.28220844, 3.83515167, -2.42638111), (1.0, 2.0, 3.0)])
+def test_bh_coincident_pair_is_dropped(oracle, c):
+    """Octree::Split (Octree.cpp:16-51) cannot separate two particles at one
+    position.  At the first coordinate the cube containing them stops
+    shrinking once it is one FP32 ulp wide (lo + size/2 rounds up), so the
+    reference's Add recursion would never end; at the second a child cube
+    comes out empty and the reference drops both.  The oracle drops both in
+    either case: the pair feels the third particle as the direct sum gives
+    it, and the third particle feels nothing (the tree holds no mass of the
+    pair)."""
+    pos = np.array([c, c, (7.5, -6.25, 8.0)], np.float32)
+    m = np.array([2.0, 3.0, 5.0])
+    for theta in (0.0, 0.5, 2.0):
+        F, st = oracle.bh_forces(pos, m, theta, [-20.0, -20.0, -20.0], 40.0, G=1.0, S=0.01)
+        assert st["dropped"] == 2 and st["leaves"] == 1
+        assert np.all(F[2] == 0)
+        for i in (0, 1):
+            pair = oracle.ref_forces(pos[[i, 2]], m[[i, 2]], G=1.0, S=0.01)
+            np.testing.assert_allclose(F[i], pair[0], rtol=1e-12)
