@@ -1049,6 +1049,12 @@ using core::kahan_add;
 // moves only the address and the lane mask.
 constexpr int kListCap = 96;
 
+// Margins of the box verdicts: 64 FP32 ulps, above the 16 that the rounding of
+// a lane's squared distance and of the box's nearest and farthest squared
+// distances can add up to.
+constexpr float kBoxLow = 1.0f - 0x1p-18f;
+constexpr float kBoxHigh = 1.0f + 0x1p-18f;
+
 // x where the lane's bit of the entry mask is set, else 0 (the bit
 // sign-extended to a full-width and-mask); w is the mask word holding the
 // lane's bit (lanes 0-31 read the low word, lanes 32-63 the high one).
@@ -1058,6 +1064,12 @@ __device__ __forceinline__ float lane_masked(float x, unsigned w, int lane) {
 
 // v_writelane_b32 through the compiler's own intrinsic is declared above the
 // namespace (nb_writelane).
+
+// A float that is equal in every lane, as a scalar register (the bits moved,
+// not converted: the builtin takes an int).
+__device__ __forceinline__ float uniform_float(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
 
 // (lo, hi) = x in lane l only (register writes; the other lanes keep lo and
 // hi).  l and x are uniform over the wave.
@@ -1149,8 +1161,11 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return ((k / C) * kXcds + x) * C + k % C;
 }
 
+// Seven waves per SIMD: 71 VGPRs (a few scalar values spill to VGPR lanes,
+// none to memory); left to itself the compiler takes 73 and six waves, which
+// measured slightly slower.
 template <int LAW>
-__global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __restrict__ pos_sorted,
+__global__ __launch_bounds__(kTravThreads) __attribute__((amdgpu_waves_per_eu(7))) void traverse_kernel(const float4* __restrict__ pos_sorted,
                                                                 const ChildBlock* __restrict__ blocks,
                                                                 const int* __restrict__ node_first,
                                                                 const int* __restrict__ node_last, TravArgs A) {
@@ -1214,6 +1229,35 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         const int l = q - wave_base;
         return (l >= 0 && l < 64) ? (m & ~(1ULL << l)) : m;
     };
+
+    // The wave's bounding box (lanes past n hold the last particle, inside it).
+    // A child cell whose nearest point of the box is beyond its threshold is
+    // approximated by every lane, one whose farthest point is within it is
+    // opened by every lane: those verdicts are taken for all 64 children of a
+    // batch at once, one child per lane, and only the others are tested lane
+    // by lane.  The margins cover the rounding of both computations (each
+    // within 16 FP32 ulps of the exact squared distance), so every verdict is
+    // the decision each lane would make.  Not used when a position is not
+    // finite.
+    float bx0 = me.x, by0 = me.y, bz0 = me.z, bx1 = me.x, by1 = me.y, bz1 = me.z;
+    for (int off = 32; off > 0; off >>= 1) {
+        bx0 = fminf(bx0, __shfl_xor(bx0, off));
+        by0 = fminf(by0, __shfl_xor(by0, off));
+        bz0 = fminf(bz0, __shfl_xor(bz0, off));
+        bx1 = fmaxf(bx1, __shfl_xor(bx1, off));
+        by1 = fmaxf(by1, __shfl_xor(by1, off));
+        bz1 = fmaxf(bz1, __shfl_xor(bz1, off));
+    }
+    const bool box_ok =
+        __builtin_amdgcn_ballot_w64(!(__builtin_isfinite(me.x) && __builtin_isfinite(me.y) &&
+                                      __builtin_isfinite(me.z))) == 0ull;
+    // uniform over the wave after the reduction: scalar registers
+    bx0 = uniform_float(bx0);
+    by0 = uniform_float(by0);
+    bz0 = uniform_float(bz0);
+    bx1 = uniform_float(bx1);
+    by1 = uniform_float(by1);
+    bz1 = uniform_float(bz1);
 
     const unsigned long long all = __builtin_amdgcn_ballot_w64(valid);
     if (all) {
@@ -1282,7 +1326,26 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         const bool in_batch = grp < k;
         const ChildBlock* VB = blocks + (in_batch ? vcell : top);
         const float4 vsrc = VB->src[lane & 7];
+        const float vthr = VB->thr[lane & 7];
         const int vcode = VB->code[lane & 7];
+        // Box verdicts for the child this lane holds (see the box above).
+        bool open_all = false, accept_all = false;
+        if (box_ok) {
+            const float ex = fmaxf(fmaxf(bx0 - vsrc.x, vsrc.x - bx1), 0.f);
+            const float ey = fmaxf(fmaxf(by0 - vsrc.y, vsrc.y - by1), 0.f);
+            const float ez = fmaxf(fmaxf(bz0 - vsrc.z, vsrc.z - bz1), 0.f);
+            const float fx = fmaxf(vsrc.x - bx0, bx1 - vsrc.x);
+            const float fy = fmaxf(vsrc.y - by0, by1 - vsrc.y);
+            const float fz = fmaxf(vsrc.z - bz0, bz1 - vsrc.z);
+            const float near2 = (ex * ex + ey * ey) + ez * ez;
+            const float far2 = (fx * fx + fy * fy) + fz * fz;
+            accept_all = near2 * kBoxLow >= vthr;
+            open_all = far2 * kBoxHigh < vthr;
+        }
+        // child cells of the batch that need the lane-by-lane test
+        const unsigned long long tested =
+            __builtin_amdgcn_ballot_w64(vcode >= 0 && !open_all && !accept_all) &
+            (k == 8 ? ~0ull : ((1ull << (8 * k)) - 1ull));
         // Decisions, cell by cell.  The whole block in scalar registers at
         // once (the empty asm keeps the compiler from sinking each slot's
         // loads into that slot's branch, which would wait on memory once per
@@ -1294,9 +1357,11 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
         // all lanes).  A block holds its child cells first, then its
         // particles, then empty codes, so the tests stop at the first code
         // below zero.
-        ge_lo = ~0u;
-        ge_hi = ~0u;
+        ge_lo = open_all ? 0u : ~0u;
+        ge_hi = open_all ? 0u : ~0u;
         for (int j = 0; j < k; ++j) {
+            const unsigned tj = static_cast<unsigned>(tested >> (8 * j)) & 0xffu;
+            if (tj == 0u) continue;  // every child cell of cell j has its box verdict
             const int code = __builtin_amdgcn_readlane(vcell, j * 8);
             const ChildBlock* B = blocks + code;
             const int4 c0 = *reinterpret_cast<const int4*>(B->code);
@@ -1316,6 +1381,7 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
             const int lbase = j * 8;
             auto decide = [&](auto S) {
                 constexpr int slot = decltype(S)::value;
+                if (!((tj >> slot) & 1u)) return;
                 const float dx = me.x - srcs[slot].x, dy = me.y - srcs[slot].y, dz = me.z - srcs[slot].z;
                 const float r2 = (dx * dx + dy * dy) + dz * dz;
                 const unsigned long long ge = __builtin_amdgcn_ballot_w64(!(r2 < thrs[slot]));
