@@ -406,6 +406,25 @@ def bh_summary(pu):
     }
 
 
+def serve_summary(pu):
+    """The served workload in short (the same Poisson stream and mix for 3 s of
+    arrivals instead of 60, drained), for the default line: aggregate
+    interactions/s and completion latency."""
+    a = argparse.Namespace(seed=0x5EED0005, rate=200.0, duration=3.0, max_seconds=120.0, memory_gib=96.0, max_bh=16,
+                           bh_ahead=1, progress="", seq_every=0, eps=0.02, dt=1e-3)
+    r = bench_serve(pu, a)
+    ld = r["load"]
+    return {
+        "workload": r["config"]["workload"],
+        "interactions_per_s": r["value"],
+        "latency_s": r["latency_s"],
+        "requests": ld["offered_requests"],
+        "completed": ld["completed"],
+        "wall_seconds": ld["wall_seconds"],
+        "note": "the full 60-s stream is `bench.py --workload serve`",
+    }
+
+
 def bench_serve(pu, args):
     """Served workload: Poisson arrivals (rate 200/s for `duration` s), sizes
     log-uniform in [2048, 524288], steps uniform in [10, 200]; below 32768
@@ -604,6 +623,7 @@ def main():
     ap.add_argument("--baseline-steps", type=int, default=10)
     ap.add_argument("--no-batched", action="store_true", help="direct: skip the batched step-latency leg")
     ap.add_argument("--no-bh", action="store_true", help="direct: skip the Barnes-Hut summary leg")
+    ap.add_argument("--no-served", action="store_true", help="direct: skip the short served-stream leg")
     args = ap.parse_args()
 
     if args.workload == "copy":
@@ -619,8 +639,9 @@ def main():
         )
         sys.exit(2)
 
-    if args.workload == "serve" and "GPU_MAX_HW_QUEUES" not in os.environ:
-        # The server overlaps one stream per Barnes-Hut request with the batch's
+    if args.workload in ("serve", "direct") and "GPU_MAX_HW_QUEUES" not in os.environ:
+        # The server (the serve workload, and the default line's short served
+        # leg) overlaps one stream per Barnes-Hut request with the batch's
         # stream; HIP's default of 4 hardware queues serialises their small
         # tree-build kernels (16: +24% throughput, p99 52.6 s instead of 67.7 s
         # on a 10-s run of this workload).  Read when the HIP runtime starts,
@@ -758,6 +779,8 @@ def main():
         result["batched"] = batch_step_latency(pu)
     if not args.no_bh and n == 1 << 20:
         result["barnes_hut"] = bh_summary(pu)
+    if not args.no_served and n == 1 << 20:
+        result["served_short"] = serve_summary(pu)
 
     if not args.no_cpu_baseline:
         # CPU-baseline leg: the CPU restatement is the checker of the first
