@@ -624,6 +624,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="direct: skip the batched step-latency leg")
     ap.add_argument("--no-bh", action="store_true", help="direct: skip the Barnes-Hut summary leg")
     ap.add_argument("--no-served", action="store_true", help="direct: skip the short served-stream leg")
+    ap.add_argument("--hw-queues", type=int, default=16, choices=range(1, 33), metavar="1..32",
+                    help="serve, direct: GPU_MAX_HW_QUEUES for this process (the server's overlapping streams)")
     args = ap.parse_args()
 
     if args.workload == "copy":
@@ -639,14 +641,14 @@ def main():
         )
         sys.exit(2)
 
-    if args.workload in ("serve", "direct") and "GPU_MAX_HW_QUEUES" not in os.environ:
+    if args.workload in ("serve", "direct"):
         # The server (the serve workload, and the default line's short served
         # leg) overlaps one stream per Barnes-Hut request with the batch's
-        # stream; HIP's default of 4 hardware queues serialises their small
-        # tree-build kernels (16: +24% throughput, p99 52.6 s instead of 67.7 s
-        # on a 10-s run of this workload).  Read when the HIP runtime starts,
-        # so it is set before the first device call.
-        os.environ["GPU_MAX_HW_QUEUES"] = "16"
+        # stream.  With HIP's default of 4 hardware queues the direct
+        # requests wait behind Barnes-Hut work (10-s stream: direct p50 1.3 s
+        # against 0.5 s with 16, the same throughput and p99).  Read when the
+        # HIP runtime starts, so it is set before the first device call.
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     pu = load_package()
     if pu.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
