@@ -635,7 +635,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(
-            "bench.py: this build measures one GPU only; --gpus must be 1 "
+            "bench.py: the library measures one GPU only; --gpus must be 1 "
             "(multi-GPU sharding is a design, not a measured path)",
             file=sys.stderr,
         )

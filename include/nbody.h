@@ -66,7 +66,7 @@ typedef enum nbody_status {
     NBODY_ERR_OUT_OF_MEMORY = 2,   /* device or host allocation failed            */
     NBODY_ERR_DEVICE = 3,          /* a HIP call failed                           */
     NBODY_ERR_NOT_INITIALISED = 4, /* step/read before init                       */
-    NBODY_ERR_UNSUPPORTED = 5,     /* valid request this build does not serve     */
+    NBODY_ERR_UNSUPPORTED = 5,     /* valid request the library does not serve    */
     NBODY_ERR_CAPACITY = 6,        /* request exceeds a fixed capacity            */
     NBODY_ERR_IO = 7               /* snapshot file could not be read or written  */
 } nbody_status;
