@@ -442,7 +442,8 @@ def bench_serve(pu, args):
     steps = rng.integers(10, 201, m)
     cfg = pu.server_config(law=pu.LAW_R, softening=args.eps ** 2, theta=0.6, bh_threshold=32768,
                            memory_budget=int(args.memory_gib * (1 << 30)), direct_capacity=64 << 20,
-                           max_active_bh=args.max_bh, dt=args.dt, bh_steps_ahead=args.bh_ahead)
+                           max_active_bh=args.max_bh, dt=args.dt, bh_steps_ahead=args.bh_ahead,
+                           bh_no_graph=1 if getattr(args, "bh_no_graph", False) else 0)
     srv = pu.Server(cfg)
     offered_direct = float(sum(int(n) * int(n) * int(s) for n, s in zip(sizes, steps) if n < 32768))
     t0 = srv.clock()
@@ -614,6 +615,8 @@ def main():
     ap.add_argument("--memory-gib", type=float, default=96.0)
     ap.add_argument("--max-bh", type=int, default=16, help="serve: concurrent Barnes-Hut requests")
     ap.add_argument("--bh-ahead", type=int, default=1, help="serve: Barnes-Hut steps enqueued per request at a time")
+    ap.add_argument("--bh-no-graph", action="store_true",
+                    help="serve: plain launches for Barnes-Hut requests instead of graph replay (for kernel traces)")
     ap.add_argument("--progress", default="", help="append serve progress lines to this file")
     ap.add_argument("--seq-every", type=int, default=40,
                     help="serve: run every k-th request alone as the sequential baseline (0 = skip)")

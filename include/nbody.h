@@ -421,7 +421,10 @@ typedef struct nbody_server_config {
     int32_t  bh_steps_ahead;   /* Barnes-Hut steps enqueued per request between checks (0: 1) */
     int32_t  inject_alloc_failures; /* test hook: the server's next k request allocations fail
                                        as a full device would (0 in production)               */
-    int32_t  reserved[6];
+    int32_t  bh_no_graph;      /* 1: Barnes-Hut requests step with plain launches instead of
+                                  replaying captured graphs (for tools that cannot follow
+                                  graph capture; the results are the same)                  */
+    int32_t  reserved[5];
 } nbody_server_config;
 
 typedef struct nbody_request_result {

@@ -209,7 +209,8 @@ class ServerConfig(ctypes.Structure):
         ("keep_results", ctypes.c_int32),
         ("bh_steps_ahead", ctypes.c_int32),
         ("inject_alloc_failures", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("bh_no_graph", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
     ]
 
 

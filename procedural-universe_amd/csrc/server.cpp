@@ -177,8 +177,8 @@ nbody_status admit(nbody_server* s, Request& r, bool* ok) {
             const nbody_status sc = allocate(s, [&] { return nbody_create(&c, &r.sim); });
             if (sc == NBODY_ERR_OUT_OF_MEMORY) return defer(s);
             NB_TRY(sc);
-            // one graph launch per step instead of about forty kernels
-            const nbody_status sg = nbody_set_graph(r.sim, 1);
+            // one graph launch per step instead of about thirty kernels
+            const nbody_status sg = nbody_set_graph(r.sim, s->cfg.bh_no_graph ? 0 : 1);
             if (sg != NBODY_OK) {
                 const std::string msg = nbody_last_error();  // nbody_destroy clears it
                 nbody_destroy(r.sim);

@@ -27,9 +27,9 @@ def test_device_plummer_matches_host_generator(gpu):
     assert np.array_equal(pb, p) and np.array_equal(vb, v)
 
 
-def _serve(pu, requests, budget=1 << 34, keep=True, threshold=3000, inject=0):
+def _serve(pu, requests, budget=1 << 34, keep=True, threshold=3000, inject=0, no_graph=0):
     cfg = pu.server_config(memory_budget=budget, direct_capacity=1 << 20, bh_threshold=threshold, keep_results=1,
-                           dt=1e-3, inject_alloc_failures=inject)
+                           dt=1e-3, inject_alloc_failures=inject, bh_no_graph=no_graph)
     out = {}
     with pu.Server(cfg) as srv:
         ids = [srv.submit_plummer(n, steps, seed) for (n, steps, seed) in requests]
@@ -156,4 +156,19 @@ def test_server_defers_requests_whose_allocation_fails(gpu):
     assert {r["method"] for r in res1} == {pu.METHOD_DIRECT, pu.METHOD_BARNES_HUT}
     for a, b in zip(ids0, ids1):
         for x, y in zip(out0[a], out1[b]):
+            assert np.array_equal(x, y)
+
+
+def test_server_plain_launches_equal_graph_replay(gpu):
+    """bh_no_graph = 1 steps the Barnes-Hut requests with plain launches: every
+    request's final state equals the graph-replayed server's bit for bit."""
+    pu = gpu
+    rng = np.random.default_rng(9)
+    reqs = [(int(n), int(s), 2000 + k) for k, (n, s) in
+            enumerate(zip(rng.integers(500, 9000, 10), rng.integers(2, 7, 10)))]
+    ids_g, res_g, _, _, _ = _serve(pu, reqs)
+    ids_p, res_p, results, _, _ = _serve(pu, reqs, no_graph=1)
+    assert pu.METHOD_BARNES_HUT in {r["method"] for r in results}
+    for a, b in zip(ids_g, ids_p):
+        for x, y in zip(res_g[a], res_p[b]):
             assert np.array_equal(x, y)
