@@ -153,3 +153,44 @@ def test_ic_invalid_arguments(pu):
         pu.ic_plummer(10, 1, a=-1.0)
     with pytest.raises(pu.NBodyError):
         pu.ic_disc_bulge(10, 1, bulge_fraction=1.5)
+
+
+def test_last_error_is_per_thread(pu):
+    """nbody_last_error() belongs to the calling thread: a failure in one
+    thread leaves another thread's message untouched (no GPU: the config is
+    rejected before any device call)."""
+    import threading
+
+    lib = pu.load_library()
+    bar = threading.Barrier(2)
+    seen = {}
+
+    def failing():
+        c = pu.default_config()
+        c.law = 7
+        h = pu._P()
+        seen["status"] = lib.nbody_create(ctypes.byref(c), ctypes.byref(h))
+        seen["a"] = lib.nbody_last_error()
+        bar.wait()
+        bar.wait()
+        seen["a_after"] = lib.nbody_last_error()
+
+    def other():
+        c = pu.default_config()
+        c.theta = -1.0
+        h = pu._P()
+        bar.wait()
+        seen["b_before"] = lib.nbody_last_error()
+        seen["b_status"] = lib.nbody_create(ctypes.byref(c), ctypes.byref(h))
+        seen["b"] = lib.nbody_last_error()
+        bar.wait()
+
+    threads = [threading.Thread(target=failing), threading.Thread(target=other)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(30)
+    assert seen["status"] == 1 and b"law" in seen["a"]  # NBODY_ERR_INVALID_ARG
+    assert not seen["b_before"]  # the other thread's failure is not visible here
+    assert seen["b_status"] == 1 and b"theta" in seen["b"]
+    assert seen["a_after"] == seen["a"]  # and this thread's failure is not visible there
