@@ -628,7 +628,8 @@ def main():
     ap.add_argument("--no-bh", action="store_true", help="direct: skip the Barnes-Hut summary leg")
     ap.add_argument("--no-served", action="store_true", help="direct: skip the short served-stream leg")
     ap.add_argument("--hw-queues", type=int, default=16, choices=range(1, 33), metavar="1..32",
-                    help="serve, direct: GPU_MAX_HW_QUEUES for this process (the server's overlapping streams)")
+                    help="serve, direct: GPU_MAX_HW_QUEUES for this process, in place of the environment's "
+                    "(the server's overlapping streams)")
     args = ap.parse_args()
 
     if args.workload == "copy":
