@@ -728,8 +728,7 @@ __global__ __launch_bounds__(256) void block_kernel(BlockArgs A) {
             }
         }
     }
-    // The child cells first, then the particles, each in key order: the walk
-    // tests the cells and stops at the first code below zero.
+    // The child cells first, then the particles, each in key order.
     ChildBlock& B = A.blocks[i];
     int slot = 0;
 #pragma unroll
@@ -1346,17 +1345,16 @@ __global__ __launch_bounds__(kTravThreads) __attribute__((amdgpu_waves_per_eu(7)
         const unsigned long long tested =
             __builtin_amdgcn_ballot_w64(vcode >= 0 && !open_all && !accept_all) &
             (k == 8 ? ~0ull : ((1ull << (8 * k)) - 1ull));
-        // Decisions, cell by cell.  The whole block in scalar registers at
-        // once (the empty asm keeps the compiler from sinking each slot's
-        // loads into that slot's branch, which would wait on memory once per
-        // cache line).  A child particle needs no test: every lane of the
-        // mask but its own takes it.  A child cell is approximated by the
-        // lanes with !(r2 < thr) (the reference's side / |p - COM| < theta,
-        // see open_threshold) and opened by the others; that lane mask goes
-        // to lane 8j + s, unless it is every lane (the lanes' masks start as
-        // all lanes).  A block holds its child cells first, then its
-        // particles, then empty codes, so the tests stop at the first code
-        // below zero.
+        // Decisions, cell by cell, for the tested children only (a cell none
+        // of whose children is tested is skipped).  The block's centres of
+        // mass and thresholds in scalar registers at once (the empty asm
+        // keeps the compiler from sinking each slot's loads into that slot's
+        // branch, which would wait on memory once per cache line).  A child
+        // particle needs no test: every lane of the mask but its own takes
+        // it.  A child cell is approximated by the lanes with !(r2 < thr) (the
+        // reference's side / |p - COM| < theta, see open_threshold) and opened
+        // by the others; that lane mask goes to lane 8j + s, unless it is
+        // every lane (the lanes' masks start as all lanes).
         ge_lo = open_all ? 0u : ~0u;
         ge_hi = open_all ? 0u : ~0u;
         for (int j = 0; j < k; ++j) {
@@ -1364,9 +1362,6 @@ __global__ __launch_bounds__(kTravThreads) __attribute__((amdgpu_waves_per_eu(7)
             if (tj == 0u) continue;  // every child cell of cell j has its box verdict
             const int code = __builtin_amdgcn_readlane(vcell, j * 8);
             const ChildBlock* B = blocks + code;
-            const int4 c0 = *reinterpret_cast<const int4*>(B->code);
-            const int4 c1 = *reinterpret_cast<const int4*>(B->code + 4);
-            int codes[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
             float4 srcs[8];
             float thrs[8];
 #pragma unroll
@@ -1376,39 +1371,17 @@ __global__ __launch_bounds__(kTravThreads) __attribute__((amdgpu_waves_per_eu(7)
             }
 #pragma unroll
             for (int slot = 0; slot < 8; ++slot)
-                asm volatile("" ::"s"(srcs[slot].x), "s"(srcs[slot].y), "s"(srcs[slot].z), "s"(srcs[slot].w),
-                             "s"(thrs[slot]), "s"(codes[slot]));
+                asm volatile("" ::"s"(srcs[slot].x), "s"(srcs[slot].y), "s"(srcs[slot].z), "s"(thrs[slot]));
             const int lbase = j * 8;
-            auto decide = [&](auto S) {
-                constexpr int slot = decltype(S)::value;
-                if (!((tj >> slot) & 1u)) return;
+            // the tested children are cells of cell j (its block holds them in
+            // its first slots): a scalar bit test per slot
+#pragma unroll
+            for (int slot = 0; slot < 8; ++slot) {
+                if (!((tj >> slot) & 1u)) continue;
                 const float dx = me.x - srcs[slot].x, dy = me.y - srcs[slot].y, dz = me.z - srcs[slot].z;
                 const float r2 = (dx * dx + dy * dy) + dz * dz;
                 const unsigned long long ge = __builtin_amdgcn_ballot_w64(!(r2 < thrs[slot]));
                 if (ge != ~0ull) write_lane64(ge_lo, ge_hi, ge, lbase + slot);
-            };
-            using std::integral_constant;
-            if (codes[0] >= 0) {
-                decide(integral_constant<int, 0>{});
-                if (codes[1] >= 0) {
-                    decide(integral_constant<int, 1>{});
-                    if (codes[2] >= 0) {
-                        decide(integral_constant<int, 2>{});
-                        if (codes[3] >= 0) {
-                            decide(integral_constant<int, 3>{});
-                            if (codes[4] >= 0) {
-                                decide(integral_constant<int, 4>{});
-                                if (codes[5] >= 0) {
-                                    decide(integral_constant<int, 5>{});
-                                    if (codes[6] >= 0) {
-                                        decide(integral_constant<int, 6>{});
-                                        if (codes[7] >= 0) decide(integral_constant<int, 7>{});
-                                    }
-                                }
-                            }
-                        }
-                    }
-                }
             }
         }
         // Lane 8j + s forms child s's two masks: approximating lanes am (for a
