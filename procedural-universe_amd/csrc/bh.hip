@@ -1064,11 +1064,6 @@ __device__ __forceinline__ float lane_masked(float x, unsigned w, int lane) {
 // v_writelane_b32 through the compiler's own intrinsic is declared above the
 // namespace (nb_writelane).
 
-// A float that is equal in every lane, as a scalar register (the bits moved,
-// not converted: the builtin takes an int).
-__device__ __forceinline__ float uniform_float(float x) {
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
-}
 
 // (lo, hi) = x in lane l only (register writes; the other lanes keep lo and
 // hi).  l and x are uniform over the wave.
@@ -1160,11 +1155,12 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return ((k / C) * kXcds + x) * C + k % C;
 }
 
-// Seven waves per SIMD: 71 VGPRs (a few scalar values spill to VGPR lanes,
-// none to memory); left to itself the compiler takes 73 and six waves, which
-// measured slightly slower.
+// 79 VGPRs: six waves per SIMD by the compiler's estimate, which is also
+// what the workgroup limit lets stay resident.  Held to seven (72 VGPRs) the
+// box's six bounds went to scalar registers and were spilled and reloaded
+// every batch: 2.3% slower.
 template <int LAW>
-__global__ __launch_bounds__(kTravThreads) __attribute__((amdgpu_waves_per_eu(7))) void traverse_kernel(const float4* __restrict__ pos_sorted,
+__global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __restrict__ pos_sorted,
                                                                 const ChildBlock* __restrict__ blocks,
                                                                 const int* __restrict__ node_first,
                                                                 const int* __restrict__ node_last, TravArgs A) {
@@ -1250,13 +1246,6 @@ __global__ __launch_bounds__(kTravThreads) __attribute__((amdgpu_waves_per_eu(7)
     const bool box_ok =
         __builtin_amdgcn_ballot_w64(!(__builtin_isfinite(me.x) && __builtin_isfinite(me.y) &&
                                       __builtin_isfinite(me.z))) == 0ull;
-    // uniform over the wave after the reduction: scalar registers
-    bx0 = uniform_float(bx0);
-    by0 = uniform_float(by0);
-    bz0 = uniform_float(bz0);
-    bx1 = uniform_float(bx1);
-    by1 = uniform_float(by1);
-    bz1 = uniform_float(bz1);
 
     const unsigned long long all = __builtin_amdgcn_ballot_w64(valid);
     if (all) {
