@@ -1088,7 +1088,8 @@ __device__ __forceinline__ void write_entry(float4* lsrc, float* lk, uint2* lmas
 }
 
 // The list evaluated by every lane, four entries at a time with grouped rsq
-// (the direct sum's pair arithmetic, direct_core.hpp), each term zeroed for
+// and the last one to three singly (no padding entries: each would cost a
+// full evaluation; the direct sum's pair arithmetic, direct_core.hpp), each term zeroed for
 // lanes outside its entry; (px, py, pz) is the lane's particle in scaled
 // units.  The sums (scaled units) go to the lane's compensated accumulator.
 template <int LAW>
@@ -1099,7 +1100,8 @@ __device__ __forceinline__ void flush_list(const float4* lsrc, const float* lk, 
     float ax = 0.f, ay = 0.f, az = 0.f;
     // the lane's word of every entry's mask
     const unsigned* words = reinterpret_cast<const unsigned*>(lmask) + (lane >> 5);
-    for (int e = 0; e < cnt; e += 4) {
+    const int cnt4 = cnt & ~3;
+    for (int e = 0; e < cnt4; e += 4) {
         float4 src[4];
         float dx[4], dy[4], dz[4], q[4];
         const float4 k4 = *reinterpret_cast<const float4*>(lk + e);
@@ -1133,6 +1135,32 @@ __device__ __forceinline__ void flush_list(const float4* lsrc, const float* lk, 
             ay = __builtin_fmaf(dy[g], f, ay);
             az = __builtin_fmaf(dz[g], f, az);
         }
+    }
+    // the last one to three entries one at a time (the same arithmetic and
+    // order; v_rsq_f32 through the compiler's builtin)
+    for (int e = cnt4; e < cnt; ++e) {
+        const float4 src = lsrc[e];
+        const float dx = src.x - px, dy = src.y - py, dz = src.z - pz;
+        float q;
+        if constexpr (LAW == NBODY_LAW_R) {
+            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            const float t = __builtin_fmaf(r2, src.w, lk[e]);
+            const float u = r2 * t;
+            q = __builtin_fmaf(u, t, tiny);
+        } else {
+            q = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, soft)));
+        }
+        q = __builtin_amdgcn_rsqf(q);
+        float f;
+        if constexpr (LAW == NBODY_LAW_R) {
+            f = q;
+        } else {
+            f = (src.w * q) * (q * q);
+        }
+        f = lane_masked(f, words[e * 2], lane);
+        ax = __builtin_fmaf(dx, f, ax);
+        ay = __builtin_fmaf(dy, f, ay);
+        az = __builtin_fmaf(dz, f, az);
     }
     kahan_add(sx, cx, ax);
     kahan_add(sy, cy, ay);
@@ -1205,13 +1233,7 @@ __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __
     };
     auto flush = [&]() {
         entries += static_cast<unsigned long long>(cnt);
-        const int pad = (4 - (cnt & 3)) & 3;
-        if (lane < pad) {  // masked for every lane: the terms are zeroed whatever they are
-            lsrc[wave][cnt + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
-            lk[wave][cnt + lane] = 0.f;
-            lmask[wave][cnt + lane] = make_uint2(0u, 0u);
-        }
-        const int total = __builtin_amdgcn_readfirstlane(cnt + pad);
+        const int total = __builtin_amdgcn_readfirstlane(cnt);
         __builtin_amdgcn_wave_barrier();
         flush_list<LAW>(lsrc[wave], lk[wave], lmask[wave], total, lane, mx, my, mz, soft, sx, sy, sz, cx, cy, cz);
         __builtin_amdgcn_wave_barrier();
