@@ -1183,10 +1183,10 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return ((k / C) * kXcds + x) * C + k % C;
 }
 
-// 79 VGPRs: six waves per SIMD by the compiler's estimate, which is also
-// what the workgroup limit lets stay resident.  Held to seven (72 VGPRs) the
-// box's six bounds went to scalar registers and were spilled and reloaded
-// every batch: 2.3% slower.
+// 72 VGPRs (70 for law P): seven waves per SIMD by registers, while the
+// workgroup limit keeps 24 one-wave workgroups per CU (six per SIMD)
+// resident.  The box's six bounds stay in vector registers: held in scalar
+// registers they were spilled and reloaded every batch (2.3% slower).
 template <int LAW>
 __global__ __launch_bounds__(kTravThreads) void traverse_kernel(const float4* __restrict__ pos_sorted,
                                                                 const ChildBlock* __restrict__ blocks,
