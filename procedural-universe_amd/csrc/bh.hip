@@ -602,15 +602,15 @@ __device__ void radix_tree_block(const TreeArgs& T, int block) {
 // Every octree cell below the bucket level writes its own child block: its
 // child cells, then its child particles, each in key order (which is octant
 // order), packed into the first slots, the remaining codes kEmpty.  A child
-// cell gets (COM, G M), its opening
-// threshold and its node; a particle (position, G m), threshold 0 (r2 >= 0
-// always holds) and its code.  The children are the first kept nodes and the
-// particles below the cell in the radix tree.  A node that is not kept has
-// its parent's level, so its common prefix ends inside the same 3-bit group
-// as the cell's; prefixes lengthen strictly from parent to child, so at most
-// two such nodes lie on any path below the cell and a descent of three
-// binary levels reaches every child.  Runs after the cells' centres of mass
-// are final; particles of a bucket are reached by its range instead.
+// cell gets (COM, G M), its opening threshold and its node; a particle
+// (position, G m), threshold 0 (r2 >= 0 always holds) and its code.  The
+// children are the first kept nodes and the particles below the cell in the
+// radix tree.  A node that is not kept has its parent's level, so its common
+// prefix ends inside the same 3-bit group as the cell's; prefixes lengthen
+// strictly from parent to child, so at most two such nodes lie on any path
+// below the cell and a descent of three binary levels reaches every child.
+// Runs after the cells' centres of mass are final; particles of a bucket are
+// reached by its range instead.
 struct BlockArgs {
     const int2* child;
     const int* info;
@@ -622,134 +622,147 @@ struct BlockArgs {
     DirectScale sc;    // the pair arithmetic's scaled units (mass factors)
 };
 
+// A workgroup takes kBlockNodes radix nodes and first gathers the cells among
+// them (kept, level below `limit`) into an LDS list in node order, so that
+// whole wavefronts work on cells (45% of the nodes on a disc encounter).
+// Returns the list's length (uniform).
+constexpr int kBlockNodes = 256;
+constexpr int kBlockCells = 4;  // cells per lane group in flight together
+constexpr int kBlockRounds = kBlockNodes / 256;
+
+__device__ inline int gather_cells(const int* __restrict__ info, int m, int base, int limit, int* list,
+                                   int* wave_cnt) {
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const unsigned long long below = (1ULL << lane) - 1ULL;
+    unsigned long long mask[kBlockRounds];
+#pragma unroll
+    for (int r = 0; r < kBlockRounds; ++r) {
+        const int node = base + r * 256 + tid;
+        const int w = node < m ? info[node] : 0;
+        mask[r] = __builtin_amdgcn_ballot_w64(info_kept(w) && info_level(w) < limit);
+        if (lane == 0) wave_cnt[r * 4 + wave] = static_cast<int>(__popcll(mask[r]));
+    }
+    __syncthreads();
+    int total = 0;
+#pragma unroll
+    for (int k = 0; k < 4 * kBlockRounds; ++k) total += wave_cnt[k];
+#pragma unroll
+    for (int r = 0; r < kBlockRounds; ++r) {
+        int at = 0;
+        for (int k = 0; k < r * 4 + wave; ++k) at += wave_cnt[k];
+        if ((mask[r] >> lane) & 1ULL) list[at + static_cast<int>(__popcll(mask[r] & below))] = base + r * 256 + tid;
+    }
+    __syncthreads();
+    return total;
+}
+
+// Eight lanes per cell.  Lane s of a cell's group follows the path (a, b, c)
+// = the bits of s down the binary levels below the cell and takes the child
+// at its end; a child met higher up (a kept node or a particle) belongs to
+// the first lane of its subtree, the others of that subtree take none.  Each
+// level's word and children are loaded together (a radix node always has two
+// children, so every load is in range).  Then each lane writes one slot of
+// the block: the lanes with child cells the first slots, then those with
+// particles, each in lane order (= key order), then the empty ones, so the
+// 192 bytes of a block go out in three stores of eight consecutive lanes.
+// Every group takes kBlockCells cells at a time (independent load chains in
+// flight together).
+template <int LAW>
+__device__ __forceinline__ void descend(const BlockArgs& A, int i, int s, int& code, int& word) {
+    // Without branches: every level's loads are issued (at node 0 where the
+    // path has ended; every index is in range), so that the cells' chains
+    // overlap, and the results are selected afterwards.
+    const int a = s >> 2, b = (s >> 1) & 1, c = s & 1;
+    const int2 top = A.child[i];
+    const int c1 = a ? top.y : top.x;
+    const int n1 = max(c1, 0);
+    const int r1 = A.info[n1];
+    const int w1 = c1 >= 0 ? r1 : 0;
+    const int2 mid = A.child[n1];
+    const int c2 = b ? mid.y : mid.x;
+    const int n2 = max(c2, 0);
+    const int r2 = A.info[n2];
+    const int w2 = c2 >= 0 ? r2 : 0;
+    const int2 low = A.child[n2];
+    const int c3 = c ? low.y : low.x;
+    const int r3 = A.info[max(c3, 0)];
+    const int w3 = c3 >= 0 ? r3 : 0;
+    const bool stop1 = c1 < 0 || info_kept(w1);
+    const bool stop2 = c2 < 0 || info_kept(w2);
+    // selects, not branches (a branch would take the deeper loads with it)
+    const int code2 = stop2 ? (c == 0 ? c2 : kEmpty) : c3;
+    const int word2 = stop2 ? (c == 0 ? w2 : 0) : w3;
+    code = stop1 ? ((b | c) == 0 ? c1 : kEmpty) : code2;
+    word = stop1 ? ((b | c) == 0 ? w1 : 0) : word2;
+}
+
+// The slot of lane s's child in its cell's block (see above) and where its
+// record is (node 0's for an empty slot, so that every load is in range).
+__device__ __forceinline__ void child_slot(const BlockArgs& A, int code, int s, int gshift, int& slot,
+                                           const float4*& at) {
+    const unsigned below = (1u << s) - 1u;
+    const bool found = code != kEmpty;
+    const bool cell = found && code >= 0;
+    const unsigned gf = static_cast<unsigned>(__builtin_amdgcn_ballot_w64(found) >> gshift) & 0xffu;
+    const unsigned gc = static_cast<unsigned>(__builtin_amdgcn_ballot_w64(cell) >> gshift) & 0xffu;
+    if (cell) slot = __popc(gc & below);
+    else if (found) slot = __popc(gc) + __popc(gf & ~gc & below);
+    else slot = __popc(gf) + __popc(~gf & below);
+    at = (found && code < 0) ? A.pos_sorted + dec_leaf(code) : A.node_com + (cell ? code : 0);
+}
+
+// Writes lane s's slot of cell i's block from the loaded record r and the
+// threshold t of the child's level.
+template <int LAW>
+__device__ __forceinline__ void write_child(const BlockArgs& A, int i, int slot, int code, int word, float4 r,
+                                            float t) {
+    const bool found = code != kEmpty;
+    const bool cell = found && code >= 0;
+    // the list's mass factor in place of G m, once per step
+    const float4 src =
+        found ? make_float4(r.x, r.y, r.z, core::mass_factor<LAW>(r.w, A.sc)) : make_float4(0.f, 0.f, 0.f, 0.f);
+    ChildBlock& B = A.blocks[i];
+    B.src[slot] = src;
+    B.thr[slot] = cell ? t : 0.f;
+    B.code[slot] = (cell && info_level(word) >= kBucketLevel) ? (code | kBucketFlag) : code;
+}
+
 template <int LAW>
 __global__ __launch_bounds__(256) void block_kernel(BlockArgs A) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= A.m) return;
-    const int self = A.info[i];
-    if (!info_kept(self) || info_level(self) >= kBucketLevel) return;
-    // The descent, one binary level at a time with every node's children and
-    // word loaded together (a node of the radix tree always has two children,
-    // so each load is in range), into fixed positions: the path (a, b, c)
-    // below the cell is position 4a + 2b + c, and a child found higher up
-    // takes the first position of its subtree.  Positions are key order.
-    int code[8], word[8];
-    bool found[8];
+    __shared__ int list[kBlockNodes];
+    __shared__ int wave_cnt[4 * kBlockRounds];
+    const int total = gather_cells(A.info, A.m, blockIdx.x * kBlockNodes, kBucketLevel, list, wave_cnt);
+    const int lane = threadIdx.x & 63, s = threadIdx.x & 7, group = threadIdx.x >> 3;
+    const int gshift = lane & ~7;  // the group's bits in a wave's ballot
+    for (int k0 = 0; k0 < total; k0 += 32 * kBlockCells) {
+        int idx[kBlockCells], code[kBlockCells], word[kBlockCells], slot[kBlockCells];
+        bool live[kBlockCells];
+        const float4* at[kBlockCells];
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
-        code[p] = kEmpty;
-        word[p] = 0;
-        found[p] = false;
-    }
-    const int2 top = A.child[i];
-    const int c1[2] = {top.x, top.y};
-    int f1[2];
-    int2 mid[2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        f1[a] = 0;
-        mid[a] = make_int2(kEmpty, kEmpty);
-        if (c1[a] >= 0) {
-            f1[a] = A.info[c1[a]];
-            mid[a] = A.child[c1[a]];
-        }
-    }
-    bool open1[2], open2[2][2];
-    int f2[2][2];
-    int2 low[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        open1[a] = c1[a] >= 0 && !info_kept(f1[a]);
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int c2 = b ? mid[a].y : mid[a].x;
-            f2[a][b] = 0;
-            low[a][b] = make_int2(kEmpty, kEmpty);
-            if (open1[a] && c2 >= 0) {
-                f2[a][b] = A.info[c2];
-                low[a][b] = A.child[c2];
-            }
-            open2[a][b] = open1[a] && c2 >= 0 && !info_kept(f2[a][b]);
-        }
-    }
-    int f3[2][2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                const int c3 = c ? low[a][b].y : low[a][b].x;
-                f3[a][b][c] = (open2[a][b] && c3 >= 0) ? A.info[c3] : 0;
-            }
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-        if (!open1[a]) {
-            code[4 * a] = c1[a];
-            word[4 * a] = f1[a];
-            found[4 * a] = true;
-            continue;
+        for (int u = 0; u < kBlockCells; ++u) {
+            const int k = k0 + 32 * u + group;
+            live[u] = k < total;
+            idx[u] = list[live[u] ? k : 0];
         }
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int p = 4 * a + 2 * b;
-            if (!open2[a][b]) {
-                code[p] = b ? mid[a].y : mid[a].x;
-                word[p] = f2[a][b];
-                found[p] = true;
-                continue;
-            }
-            code[p] = low[a][b].x;
-            word[p] = f3[a][b][0];
-            found[p] = true;
-            code[p + 1] = low[a][b].y;
-            word[p + 1] = f3[a][b][1];
-            found[p + 1] = true;
-        }
-    }
-    // Every child's record loaded before any is written (the block may not
-    // alias the sources, but the compiler cannot know it).
-    float4 src[8];
-    float th[8];
+        for (int u = 0; u < kBlockCells; ++u) descend<LAW>(A, idx[u], s, code[u], word[u]);
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
-        src[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-        th[p] = 0.f;
-        if (found[p]) {
-            const int c = code[p];
-            const float4* at = c < 0 ? A.pos_sorted + dec_leaf(c) : A.node_com + c;
-            src[p] = *at;
-            // the list's mass factor in place of G m, once per step
-            src[p].w = core::mass_factor<LAW>(src[p].w, A.sc);
-            if (c >= 0) {
-                const int lv = info_level(word[p]);
-                th[p] = A.thr[lv];
-                if (lv >= kBucketLevel) code[p] = c | kBucketFlag;
-            }
-        }
-    }
-    // The child cells first, then the particles, each in key order.
-    ChildBlock& B = A.blocks[i];
-    int slot = 0;
+        for (int u = 0; u < kBlockCells; ++u) child_slot(A, code[u], s, gshift, slot[u], at[u]);
+        // every cell's records and thresholds in flight together
+        float4 r[kBlockCells];
+        float t[kBlockCells];
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
-        if (found[p] && code[p] >= 0) {
-            B.src[slot] = src[p];
-            B.thr[slot] = th[p];
-            B.code[slot] = code[p];
-            ++slot;
+        for (int u = 0; u < kBlockCells; ++u) {
+            r[u] = *at[u];
+            t[u] = A.thr[info_level(word[u])];
         }
-    }
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
-        if (found[p] && code[p] < 0) {
-            B.src[slot] = src[p];
-            B.thr[slot] = th[p];
-            B.code[slot] = code[p];
-            ++slot;
-        }
+        for (int u = 0; u < kBlockCells; ++u)
+            asm volatile("" ::"v"(r[u].x), "v"(r[u].y), "v"(r[u].z), "v"(r[u].w), "v"(t[u]));
+#pragma unroll
+        for (int u = 0; u < kBlockCells; ++u)
+            if (live[u]) write_child<LAW>(A, idx[u], slot[u], code[u], word[u], r[u], t[u]);
     }
-    for (int s2 = slot; s2 < 8; ++s2) B.code[s2] = kEmpty;
 }
 
 // ---- 5. FP64 mass and centre of mass ------------------------------------------------
@@ -1752,9 +1765,9 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     BlockArgs L{b->child, b->info, b->com, b->thr, b->pos_sorted, m, b->blocks, s->dscale};
     b->built = true;
     if (s->cfg.law == NBODY_LAW_R)
-        hipLaunchKernelGGL(block_kernel<NBODY_LAW_R>, dim3(blocks_for(m, 256)), dim3(256), 0, st, L);
+        hipLaunchKernelGGL(block_kernel<NBODY_LAW_R>, dim3(blocks_for(m, kBlockNodes)), dim3(256), 0, st, L);
     else
-        hipLaunchKernelGGL(block_kernel<NBODY_LAW_P>, dim3(blocks_for(m, 256)), dim3(256), 0, st, L);
+        hipLaunchKernelGGL(block_kernel<NBODY_LAW_P>, dim3(blocks_for(m, kBlockNodes)), dim3(256), 0, st, L);
     NB_HIP(hipGetLastError());
     return NBODY_OK;
 }
