@@ -847,20 +847,33 @@ constexpr int kMassWaves = kMassThreads / 64;
 // kMassThreads consecutive elements, read with coalesced loads.  A row is
 // scanned by wave scans plus the totals of the waves before it.
 
-// Sum of each chunk, per quantity (rows in order, a row's waves in order).
+// Sum of each chunk, per quantity: each thread adds its kMassPer elements in
+// order (rows of kMassThreads consecutive elements, so every row's loads are
+// coalesced), then each wave's lanes are added towards lane 0 (a fixed order)
+// and thread q adds the waves' sums in order.  The chunk sums only carry the
+// prefix from one chunk into the next; their rounding (double-double, about
+// 2^-106 of the total) is far below FP64 resolution of any node's sum.
 __device__ void mass_reduce_block(const MassArgs& A, int block) {
     __shared__ DD wsum[4][kMassWaves];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     DD acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    // four rows' loads in flight at a time: fully unrolled, the loads of
+    // every row were held in registers (144 VGPRs) and the launch's occupancy,
+    // the tree's workgroups included, fell to three waves per SIMD
+#pragma unroll 4
     for (int r = 0; r < kMassPer; ++r) {
         const int i = block * kMassChunk + r * kMassThreads + threadIdx.x;
         double v[4];
         leaf_vals(A, i, v);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const DD incl = wave_scan_dd(DD{v[q], 0.0}, lane);
-            const DD tot{__shfl(incl.hi, 63, 64), __shfl(incl.lo, 63, 64)};
-            acc[q] = dd_add(acc[q], tot);
+        for (int q = 0; q < 4; ++q) acc[q] = dd_add(acc[q], DD{v[q], 0.0});
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const DD o{__shfl_down(acc[q].hi, off, 64), __shfl_down(acc[q].lo, off, 64)};
+            acc[q] = dd_add(acc[q], o);
         }
     }
     if (lane == 0)
@@ -874,12 +887,14 @@ __device__ void mass_reduce_block(const MassArgs& A, int block) {
 }
 
 // The radix tree's nodes and the chunk sums of the mass prefix are
-// independent: one launch, the first `tree_blocks` workgroups build nodes and
-// the others reduce chunks (both read only the sorted particles).
+// independent: one launch, the first A.nchunks workgroups reduce chunks (long
+// serial chains, started first so that the tree's many short workgroups run
+// beside them) and the others build nodes (both read only the sorted
+// particles).
 static_assert(kMassThreads == 256, "one workgroup size for both halves");
-__global__ __launch_bounds__(256) void tree_mass_kernel(TreeArgs T, MassArgs A, int tree_blocks) {
-    if (static_cast<int>(blockIdx.x) < tree_blocks) radix_tree_block(T, blockIdx.x);
-    else mass_reduce_block(A, blockIdx.x - tree_blocks);
+__global__ __launch_bounds__(256) void tree_mass_kernel(TreeArgs T, MassArgs A) {
+    if (static_cast<int>(blockIdx.x) < A.nchunks) mass_reduce_block(A, blockIdx.x);
+    else radix_tree_block(T, blockIdx.x - A.nchunks);
 }
 
 // Exclusive scan of the chunk sums (one workgroup, quantities in turn).
@@ -1753,7 +1768,7 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     const int tree_blocks = blocks_for(m, 256);
     const int mchunks = blocks_for(n + 1, kMassChunk);
     MassArgs M{b->pos_sorted, order, s->mass_dev, n, b->prefix, b->mass_chunk, mchunks};
-    hipLaunchKernelGGL(tree_mass_kernel, dim3(tree_blocks + mchunks), dim3(256), 0, st, T, M, tree_blocks);
+    hipLaunchKernelGGL(tree_mass_kernel, dim3(mchunks + tree_blocks), dim3(256), 0, st, T, M);
     if (mchunks <= kMassFusedChunks) {
         hipLaunchKernelGGL(mass_prefix_kernel<true>, dim3(mchunks), dim3(kMassThreads), 0, st, M);
     } else {
