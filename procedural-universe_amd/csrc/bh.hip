@@ -794,6 +794,7 @@ __device__ inline double dd_diff(DD a, DD b) {  // a - b rounded to double
 
 constexpr int kMassThreads = 256;
 constexpr int kMassPer = 16;
+
 constexpr int kMassChunk = kMassThreads * kMassPer;
 
 struct MassArgs {
@@ -948,14 +949,29 @@ __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
     } else {
         if (threadIdx.x < 4) carry_sh[threadIdx.x] = A.chunk[threadIdx.x * A.nchunks + blockIdx.x];
     }
-    for (int r = 0; r < kMassPer; ++r) {
-        const int i = blockIdx.x * kMassChunk + r * kMassThreads + threadIdx.x;
-        double v[4];
-        leaf_vals(A, i, v);
+    // Rows of kMassThreads x kItems elements, kItems consecutive ones per
+    // lane (a wave reads and writes contiguous runs): the lane adds its
+    // elements in order, one wave scan of the lanes' sums per row and
+    // quantity, then each element's prefix is the lane's running sum.  Four
+    // per lane take a quarter of the wave scans (small systems' prefix pass is
+    // bound by them: tree build -10% at 131,072 particles); above
+    // kMassFusedChunks chunks the lanes' 64-byte strided writes cost more than
+    // the scans save (+3% at 4,194,304), and one per lane is a plain row scan.
+    constexpr int kItems = FUSED ? 4 : 1;
+    static_assert(kMassPer % kItems == 0, "whole rows per chunk");
+#pragma unroll 1
+    for (int r = 0; r < kMassPer / kItems; ++r) {
+        const int i0 = blockIdx.x * kMassChunk + (r * kMassThreads + threadIdx.x) * kItems;
+        double v[kItems][4];
+#pragma unroll
+        for (int e = 0; e < kItems; ++e) leaf_vals(A, i0 + e, v[e]);
         DD incl[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            incl[q] = wave_scan_dd(DD{v[q], 0.0}, lane);
+            DD sum{v[0][q], 0.0};
+#pragma unroll
+            for (int e = 1; e < kItems; ++e) sum = dd_add(sum, DD{v[e][q], 0.0});
+            incl[q] = wave_scan_dd(sum, lane);
             if (lane == 63) wsum[q][wave] = incl[q];
         }
         __syncthreads();
@@ -973,8 +989,12 @@ __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const DD ex = wave_exclusive(incl[q], lane);
-            if (i <= A.n) A.prefix[q * stride + i] = dd_add(wpre[q][wave], ex);
+            DD p = dd_add(wpre[q][wave], wave_exclusive(incl[q], lane));
+#pragma unroll
+            for (int e = 0; e < kItems; ++e) {
+                if (i0 + e <= A.n) A.prefix[q * stride + i0 + e] = p;
+                p = dd_add(p, DD{v[e][q], 0.0});
+            }
         }
     }
 }
