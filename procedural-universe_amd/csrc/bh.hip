@@ -794,6 +794,8 @@ __device__ inline double dd_diff(DD a, DD b) {  // a - b rounded to double
 
 constexpr int kMassThreads = 256;
 constexpr int kMassPer = 16;
+constexpr int kMassItems = 4;  // consecutive elements per lane in the prefix pass
+static_assert(kMassPer % kMassItems == 0, "whole rows per chunk");
 
 constexpr int kMassChunk = kMassThreads * kMassPer;
 
@@ -898,27 +900,26 @@ __global__ __launch_bounds__(256) void tree_mass_kernel(TreeArgs T, MassArgs A) 
     else radix_tree_block(T, blockIdx.x - A.nchunks);
 }
 
-// Exclusive scan of the chunk sums (one workgroup, quantities in turn).
+// Exclusive scan of the chunk sums (workgroup q scans quantity q).
 __global__ __launch_bounds__(kMassThreads) void mass_chunks_kernel(MassArgs A) {
     __shared__ DD wsum[kMassWaves];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int q = 0; q < 4; ++q) {
-        DD carry{0.0, 0.0};
-        for (int b = 0; b < A.nchunks; b += kMassThreads) {
-            const int i = b + threadIdx.x;
-            const DD v = i < A.nchunks ? A.chunk[q * A.nchunks + i] : DD{0.0, 0.0};
-            const DD incl = wave_scan_dd(v, lane);
-            if (lane == 63) wsum[wave] = incl;
-            __syncthreads();
-            DD before = carry;
-            for (int w = 0; w < wave; ++w) before = dd_add(before, wsum[w]);
-            DD row = carry;
-            for (int w = 0; w < kMassWaves; ++w) row = dd_add(row, wsum[w]);
-            __syncthreads();
-            const DD excl = dd_add(before, wave_exclusive(incl, lane));
-            if (i < A.nchunks) A.chunk[q * A.nchunks + i] = excl;
-            carry = row;
-        }
+    const int q = blockIdx.x;
+    DD carry{0.0, 0.0};
+    for (int b = 0; b < A.nchunks; b += kMassThreads) {
+        const int i = b + threadIdx.x;
+        const DD v = i < A.nchunks ? A.chunk[q * A.nchunks + i] : DD{0.0, 0.0};
+        const DD incl = wave_scan_dd(v, lane);
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        DD before = carry;
+        for (int w = 0; w < wave; ++w) before = dd_add(before, wsum[w]);
+        DD row = carry;
+        for (int w = 0; w < kMassWaves; ++w) row = dd_add(row, wsum[w]);
+        __syncthreads();
+        const DD excl = dd_add(before, wave_exclusive(incl, lane));
+        if (i < A.nchunks) A.chunk[q * A.nchunks + i] = excl;
+        carry = row;
     }
 }
 
@@ -934,6 +935,7 @@ __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
     __shared__ DD wsum[4][kMassWaves];
     __shared__ DD wpre[4][kMassWaves];  // the row's sum before each wave, carry included
     __shared__ DD carry_sh[4];          // the sum before the row
+    __shared__ DD stage[kMassWaves][64 * kMassItems];  // a wave's prefixes of one quantity
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t stride = static_cast<size_t>(A.n) + 1;
     if constexpr (FUSED) {
@@ -950,15 +952,14 @@ __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
         if (threadIdx.x < 4) carry_sh[threadIdx.x] = A.chunk[threadIdx.x * A.nchunks + blockIdx.x];
     }
     // Rows of kMassThreads x kItems elements, kItems consecutive ones per
-    // lane (a wave reads and writes contiguous runs): the lane adds its
-    // elements in order, one wave scan of the lanes' sums per row and
-    // quantity, then each element's prefix is the lane's running sum.  Four
-    // per lane take a quarter of the wave scans (small systems' prefix pass is
-    // bound by them: tree build -10% at 131,072 particles); above
-    // kMassFusedChunks chunks the lanes' 64-byte strided writes cost more than
-    // the scans save (+3% at 4,194,304), and one per lane is a plain row scan.
-    constexpr int kItems = FUSED ? 4 : 1;
-    static_assert(kMassPer % kItems == 0, "whole rows per chunk");
+    // lane (a wave reads a contiguous run): the lane adds its elements in
+    // order, one wave scan of the lanes' sums per row and quantity, then each
+    // element's prefix is the lane's running sum.  Four per lane (a quarter of
+    // the wave scans, the wave's prefixes written through LDS as contiguous
+    // runs) up to kMassFusedChunks chunks, where the scans bound the pass
+    // (tree build -10% at 131,072 particles); beyond, one per lane, a plain
+    // row scan, is faster (134 against 155 us at N = 4M).
+    constexpr int kItems = FUSED ? kMassItems : 1;
 #pragma unroll 1
     for (int r = 0; r < kMassPer / kItems; ++r) {
         const int i0 = blockIdx.x * kMassChunk + (r * kMassThreads + threadIdx.x) * kItems;
@@ -987,14 +988,26 @@ __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
             carry_sh[q] = acc;
         }
         __syncthreads();
+        const int wave0 = blockIdx.x * kMassChunk + (r * kMassThreads + wave * 64) * kItems;  // the wave's run
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             DD p = dd_add(wpre[q][wave], wave_exclusive(incl[q], lane));
+            if constexpr (kItems == 1) {
+                if (wave0 + lane <= A.n) A.prefix[q * stride + wave0 + lane] = p;
+                continue;
+            }
 #pragma unroll
             for (int e = 0; e < kItems; ++e) {
-                if (i0 + e <= A.n) A.prefix[q * stride + i0 + e] = p;
+                stage[wave][lane * kItems + e] = p;
                 p = dd_add(p, DD{v[e][q], 0.0});
             }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) {
+                const int at = k * 64 + lane;
+                if (wave0 + at <= A.n) A.prefix[q * stride + wave0 + at] = stage[wave][at];
+            }
+            __builtin_amdgcn_wave_barrier();
         }
     }
 }
@@ -1792,7 +1805,7 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     if (mchunks <= kMassFusedChunks) {
         hipLaunchKernelGGL(mass_prefix_kernel<true>, dim3(mchunks), dim3(kMassThreads), 0, st, M);
     } else {
-        hipLaunchKernelGGL(mass_chunks_kernel, dim3(1), dim3(kMassThreads), 0, st, M);
+        hipLaunchKernelGGL(mass_chunks_kernel, dim3(4), dim3(kMassThreads), 0, st, M);
         hipLaunchKernelGGL(mass_prefix_kernel<false>, dim3(mchunks), dim3(kMassThreads), 0, st, M);
     }
     hipLaunchKernelGGL(cell_com_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, b->prefix, n, b->first, b->last,
