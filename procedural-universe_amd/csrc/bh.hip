@@ -148,12 +148,25 @@ __device__ inline unsigned long long spread3(unsigned long long v) {
 }
 
 // Exact floor((x - lo) / cell) for a root corner that is a multiple of the cell.
-__device__ inline long long cell_index(float x, const RootParams& R, int a) {
+// floor(k / odd) for |k| <= 2^40 and 1 <= odd < 2^24: a dynamic root's odd
+// part is 1 (no division); otherwise the quotient rounded from a double
+// product (within 2^-12 of the exact one) corrected by the exact remainder,
+// in place of a 64-bit integer division.
+__device__ inline long long floor_div_odd(long long k, int odd, double inv_odd) {
+    if (odd == 1) return k;
+    long long q = static_cast<long long>(floor(static_cast<double>(k) * inv_odd));
+    const long long r = k - q * odd;
+    if (r < 0) --q;
+    else if (r >= odd) ++q;
+    return q;
+}
+
+__device__ inline long long cell_index(float x, const RootParams& R, int a, double inv_odd) {
     float X = x * R.scale2;  // power-of-two scaling: exact
     X = fmaxf(fminf(X, 0x1p40f), -0x1p40f);
     long long k = static_cast<long long>(floorf(X));
     if (X == 0.f && x < 0.f) k = -1;
-    return floor_div_ll(k, R.odd) - R.base[a];
+    return floor_div_odd(k, R.odd, inv_odd) - R.base[a];
 }
 
 // ---- opening thresholds ------------------------------------------------------
@@ -284,12 +297,13 @@ __global__ __launch_bounds__(256) void keys_kernel(const float4* __restrict__ po
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const RootParams R = *Rp;
+    const double inv_odd = 1.0 / static_cast<double>(R.odd);
     const float4 p = pos[i];
     const float c[3] = {p.x, p.y, p.z};
     unsigned long long key = 0;
     bool out = false;
     for (int a = 0; a < 3; ++a) {
-        long long k = cell_index(c[a], R, a);
+        long long k = cell_index(c[a], R, a, inv_odd);
         if (k < 0) k = 0, out = true;
         if (k > 0x1fffff) k = 0x1fffff, out = true;
         key |= spread3(static_cast<unsigned long long>(k)) << a;
