@@ -315,21 +315,32 @@ __global__ __launch_bounds__(256) void keys_kernel(const float4* __restrict__ po
 
 // ---- 3. radix sort -----------------------------------------------------------
 
+// Digits of kRadixBits bits: kRadixPasses passes cover the 63-bit keys.
+// Nine-bit digits (seven passes) were slower from 131,072 keys up: a tile's
+// digit runs are half as long, so the scatter's writes coalesce less
+// (46.5 against 33 us per pass at 4,194,304 keys).
+constexpr int kRadixBits = 8;
+constexpr int kRadixBins = 1 << kRadixBits;
+constexpr unsigned kRadixMask = kRadixBins - 1;
+constexpr int kRadixPasses = (63 + kRadixBits - 1) / kRadixBits;
+constexpr int kDigitsPerThread = kRadixBins / kSortThreads;  // the digit phases' digits per thread
+static_assert(kRadixBins % kSortThreads == 0, "whole digits per thread");
+
 // Digit counts of each tile, digit-major (hist[d * nblocks + tile]).
 template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const unsigned long long* __restrict__ keys, int n,
                                                                   int shift, unsigned* __restrict__ hist,
                                                                   int nblocks) {
-    __shared__ unsigned h[256];
-    h[threadIdx.x] = 0;
+    __shared__ unsigned h[kRadixBins];
+    for (int d = threadIdx.x; d < kRadixBins; d += kSortThreads) h[d] = 0;
     __syncthreads();
     const int base = blockIdx.x * kSortThreads * ITEMS;
     for (int k = 0; k < ITEMS; ++k) {
         const int i = base + k * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & kRadixMask], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    for (int d = threadIdx.x; d < kRadixBins; d += kSortThreads) hist[d * nblocks + blockIdx.x] = h[d];
 }
 
 constexpr int kScanThreads2 = 256;
@@ -403,8 +414,8 @@ __global__ void publish_counters_kernel(const unsigned* __restrict__ counters,
 // One radix pass over a tile of 256 x ITEMS keys, stable.  Each wavefront owns
 // a contiguous share of the tile (ITEMS rounds of 64 consecutive keys) and
 // ranks every key among the earlier keys of its wavefront with the same digit
-// (the lanes with equal digits found by 8 ballots, running per-digit counts in
-// LDS; no workgroup barrier).  One pass over the 256 digits then gives each
+// (the lanes with equal digits found by kRadixBits ballots, running per-digit
+// counts in LDS; no workgroup barrier).  One pass over the digits then gives each
 // wavefront's offset within the tile's digit run and the runs' starts; the
 // keys are placed in LDS in digit order and written out in that order, so
 // consecutive threads write consecutive addresses within a digit's run of the
@@ -432,9 +443,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const float4* __restrict__ pos_in, float4* __restrict__ pos_out) {
     constexpr int kSortTile = kSortThreads * ITEMS;
     constexpr int kSortPerWave = 64 * ITEMS;  // a wavefront's share: ITEMS rounds of 64 keys
-    __shared__ unsigned cnt[kSortWaves][256];  // per-wave digit counts, then per-wave offsets in the run
-    __shared__ unsigned run_start[256];        // the tile's digit runs
-    __shared__ unsigned dst_base[256];         // the runs' places in the output
+    __shared__ unsigned cnt[kSortWaves][kRadixBins];  // per-wave digit counts, then per-wave offsets in the run
+    __shared__ unsigned run_start[kRadixBins];        // the tile's digit runs
+    __shared__ unsigned dst_base[kRadixBins];         // the runs' places in the output
     __shared__ unsigned wave_sum[2][kSortWaves];
     __shared__ unsigned long long skeys[kSortTile];
     __shared__ unsigned sidx[kSortTile];
@@ -443,7 +454,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const int base = blockIdx.x * kSortTile;
     const int wbase = base + wave * kSortPerWave;
 #pragma unroll
-    for (int d = lane; d < 256; d += 64) cnt[wave][d] = 0u;  // this wave's counters only
+    for (int d = lane; d < kRadixBins; d += 64) cnt[wave][d] = 0u;  // this wave's counters only
     unsigned long long key[ITEMS];
     unsigned idx[ITEMS], rank[ITEMS];
 #pragma unroll
@@ -452,10 +463,10 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
         const bool valid = i < n;
         key[r] = valid ? keys_in[i] : 0ULL;
         idx[r] = valid ? idx_in[i] : 0u;
-        const unsigned digit = static_cast<unsigned>((key[r] >> shift) & 255u);
+        const unsigned digit = static_cast<unsigned>((key[r] >> shift) & kRadixMask);
         unsigned long long peers = __builtin_amdgcn_ballot_w64(valid);
 #pragma unroll
-        for (int bit = 0; bit < 8; ++bit) {
+        for (int bit = 0; bit < kRadixBits; ++bit) {
             const bool set = (digit >> bit) & 1u;
             const unsigned long long bm = __builtin_amdgcn_ballot_w64(valid && set);
             peers &= set ? bm : ~bm;
@@ -469,31 +480,44 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     }
     __syncthreads();
     {
-        // digit tid: offsets of the wavefronts' shares within its run, the
-        // run's length, then the runs' starts (an exclusive scan over digits)
-        unsigned run = 0;
+        // Thread tid's kDigitsPerThread consecutive digits: offsets of the
+        // wavefronts' shares within each digit's run and the run's length,
+        // then the runs' starts (an exclusive scan over digits)
+        constexpr int D = kDigitsPerThread;
+        unsigned run[D], before[D], total[D];
+        unsigned rsum = 0, tsum = 0;
 #pragma unroll
-        for (int w = 0; w < kSortWaves; ++w) {
-            const unsigned c = cnt[w][tid];
-            cnt[w][tid] = run;
-            run += c;
-        }
-        // digit tid's place in the output: the digit's start (the totals of
-        // the smaller digits) plus the tile's offset among the digit's keys,
-        // both formed here from the raw counts or given by the row scan
-        unsigned before = 0, total = 0;
-        if constexpr (FUSED) {
-            const unsigned* row = counts + static_cast<size_t>(tid) * nblocks;
-            for (int b = 0; b < nblocks; ++b) {
-                const unsigned c = row[b];
-                before += b < static_cast<int>(blockIdx.x) ? c : 0u;
-                total += c;
+        for (int j = 0; j < D; ++j) {
+            const int d = tid * D + j;
+            unsigned acc = 0;
+#pragma unroll
+            for (int w = 0; w < kSortWaves; ++w) {
+                const unsigned c = cnt[w][d];
+                cnt[w][d] = acc;
+                acc += c;
             }
-        } else {
-            before = counts[tid * nblocks + blockIdx.x];
-            total = totals[tid];
+            run[j] = acc;
+            // digit d's place in the output: the digit's start (the totals of
+            // the smaller digits) plus the tile's offset among the digit's
+            // keys, both formed here from the raw counts or given by the row
+            // scan
+            before[j] = 0;
+            total[j] = 0;
+            if constexpr (FUSED) {
+                const unsigned* row = counts + static_cast<size_t>(d) * nblocks;
+                for (int b = 0; b < nblocks; ++b) {
+                    const unsigned c = row[b];
+                    before[j] += b < static_cast<int>(blockIdx.x) ? c : 0u;
+                    total[j] += c;
+                }
+            } else {
+                before[j] = counts[d * nblocks + blockIdx.x];
+                total[j] = totals[d];
+            }
+            rsum += run[j];
+            tsum += total[j];
         }
-        unsigned x = run, y = total;
+        unsigned x = rsum, y = tsum;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             const unsigned xs = __shfl_up(x, off, 64);
@@ -508,20 +532,26 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
             wave_sum[1][wave] = y;
         }
         __syncthreads();
-        unsigned start = x - run, digit_start = y - total;
+        unsigned start = x - rsum, digit_start = y - tsum;
         for (int w = 0; w < wave; ++w) {
             start += wave_sum[0][w];
             digit_start += wave_sum[1][w];
         }
-        run_start[tid] = start;
-        dst_base[tid] = digit_start + before;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            const int d = tid * D + j;
+            run_start[d] = start;
+            dst_base[d] = digit_start + before[j];
+            start += run[j];
+            digit_start += total[j];
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
         const int i = wbase + r * 64 + lane;
         if (i < n) {
-            const unsigned digit = static_cast<unsigned>((key[r] >> shift) & 255u);
+            const unsigned digit = static_cast<unsigned>((key[r] >> shift) & kRadixMask);
             const unsigned at = run_start[digit] + cnt[wave][digit] + rank[r];
             skeys[at] = key[r];
             sidx[at] = idx[r];
@@ -534,7 +564,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
         const int at = r * kSortThreads + tid;
         if (at < count) {
             const unsigned long long k = skeys[at];
-            const unsigned digit = static_cast<unsigned>((k >> shift) & 255u);
+            const unsigned digit = static_cast<unsigned>((k >> shift) & kRadixMask);
             const unsigned dst = dst_base[digit] + (static_cast<unsigned>(at) - run_start[digit]);
             keys_out[dst] = k;
             idx_out[dst] = sidx[at];
@@ -1729,15 +1759,15 @@ nbody_status bh_resize(BHState* b, size_t n, nbody_sim* s) {
     NB_TRY(bh_alloc(b->idx[0], n));
     NB_TRY(bh_alloc(b->idx[1], n));
     const size_t nb = sort_max_tiles(n);
-    NB_TRY(bh_alloc(b->hist, 256 * nb));
-    b->hist_elems = 256 * nb;
+    NB_TRY(bh_alloc(b->hist, kRadixBins * nb));
+    b->hist_elems = kRadixBins * nb;
     NB_TRY(bh_alloc(b->pos_sorted, n));
     NB_TRY(bh_alloc(b->child, m));
     NB_TRY(bh_alloc(b->first, m));
     NB_TRY(bh_alloc(b->last, m));
     NB_TRY(bh_alloc(b->info, m));
     NB_TRY(bh_alloc(b->blocks, m));
-    NB_TRY(bh_alloc(b->digit_totals, 256));
+    NB_TRY(bh_alloc(b->digit_totals, kRadixBins));
     NB_TRY(bh_alloc(b->prefix, 4 * (n + 1)));
     NB_TRY(bh_alloc(b->mass_chunk, 4 * static_cast<size_t>(blocks_for(n + 1, kMassChunk))));
     NB_TRY(bh_alloc(b->com, m));
@@ -1763,23 +1793,23 @@ nbody_status bh_set_theta(nbody_sim* s) {
     return NBODY_OK;
 }
 
-// The eight LSD passes over the keys in tiles of 256 x ITEMS (the last pass
+// The kRadixPasses LSD passes over the keys in tiles of 256 x ITEMS (the last pass
 // also gathers the sorted positions); the sorted buffer is b->sorted_buf.
 template <int ITEMS>
 void radix_sort_passes(BHState* b, int n, const float4* pos, hipStream_t st) {
     const int nb = blocks_for(n, kSortThreads * ITEMS);
     int cur = 0;
-    for (int pass = 0; pass < 8; ++pass) {
-        const int shift = 8 * pass;
+    for (int pass = 0; pass < kRadixPasses; ++pass) {
+        const int shift = kRadixBits * pass;
         // Above kFusedScanTiles tiles the tile offsets within each digit and
         // the digits' totals come from a separate scan of the counts.
         const bool fused = nb <= kFusedScanTiles;
         hipLaunchKernelGGL(radix_hist_kernel<ITEMS>, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], n, shift, b->hist, nb);
         if (!fused)
-            hipLaunchKernelGGL(radix_rowscan_kernel, dim3(256), dim3(kScanThreads2), 0, st, b->hist, nb,
+            hipLaunchKernelGGL(radix_rowscan_kernel, dim3(kRadixBins), dim3(kScanThreads2), 0, st, b->hist, nb,
                                b->digit_totals);
         // the last pass also writes the sorted positions
-        auto scatter = pass < 7 ? (fused ? radix_scatter_kernel<true, false, ITEMS> : radix_scatter_kernel<false, false, ITEMS>)
+        auto scatter = pass < kRadixPasses - 1 ? (fused ? radix_scatter_kernel<true, false, ITEMS> : radix_scatter_kernel<false, false, ITEMS>)
                                 : (fused ? radix_scatter_kernel<true, true, ITEMS> : radix_scatter_kernel<false, true, ITEMS>);
         hipLaunchKernelGGL(scatter, dim3(nb), dim3(kSortThreads), 0, st, b->keys[cur], b->idx[cur], b->keys[cur ^ 1],
                            b->idx[cur ^ 1], n, shift, b->hist, b->digit_totals, nb, pos, b->pos_sorted);
