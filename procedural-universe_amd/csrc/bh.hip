@@ -847,18 +847,29 @@ struct MassArgs {
     const float4* pos_sorted;
     const unsigned* order;  // sorted position -> particle
     const double* mass;     // by particle
+    double* mass_sorted;    // by sorted position: written by the chunk sums, read by the prefix pass
     int n;
     DD* prefix;             // [4][n + 1]
     DD* chunk;              // [4][nchunks]
     int nchunks;
 };
 
+// Element k's mass and mass moments.  The chunk sums (GATHER) read the mass
+// through the sorted order and store it in sorted order; the prefix pass then
+// reads it contiguously instead of gathering it a second time.
+template <bool GATHER>
 __device__ inline void leaf_vals(const MassArgs& A, int k, double v[4]) {
     if (k >= A.n) {
         v[0] = v[1] = v[2] = v[3] = 0.0;
         return;
     }
-    const double m = A.mass[A.order[k]];
+    double m;
+    if constexpr (GATHER) {
+        m = A.mass[A.order[k]];
+        A.mass_sorted[k] = m;
+    } else {
+        m = A.mass_sorted[k];
+    }
     const float4 p = A.pos_sorted[k];
     v[0] = m;
     v[1] = m * static_cast<double>(p.x);
@@ -911,7 +922,7 @@ __device__ void mass_reduce_block(const MassArgs& A, int block) {
     for (int r = 0; r < kMassPer; ++r) {
         const int i = block * kMassChunk + r * kMassThreads + threadIdx.x;
         double v[4];
-        leaf_vals(A, i, v);
+        leaf_vals<true>(A, i, v);
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = dd_add(acc[q], DD{v[q], 0.0});
     }
@@ -1007,7 +1018,7 @@ __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
         const int i0 = blockIdx.x * kMassChunk + (r * kMassThreads + threadIdx.x) * kItems;
         double v[kItems][4];
 #pragma unroll
-        for (int e = 0; e < kItems; ++e) leaf_vals(A, i0 + e, v[e]);
+        for (int e = 0; e < kItems; ++e) leaf_vals<false>(A, i0 + e, v[e]);
         DD incl[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -1842,7 +1853,9 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     TreeArgs T{keys, n, b->child, b->first, b->last, b->info};
     const int tree_blocks = blocks_for(m, 256);
     const int mchunks = blocks_for(n + 1, kMassChunk);
-    MassArgs M{b->pos_sorted, order, s->mass_dev, n, b->prefix, b->mass_chunk, mchunks};
+    // the sorted masses go to the sort's other key buffer, idle until the next step's keys
+    double* mass_sorted = reinterpret_cast<double*>(b->keys[b->sorted_buf ^ 1]);
+    MassArgs M{b->pos_sorted, order, s->mass_dev, mass_sorted, n, b->prefix, b->mass_chunk, mchunks};
     hipLaunchKernelGGL(tree_mass_kernel, dim3(mchunks + tree_blocks), dim3(256), 0, st, T, M);
     if (mchunks <= kMassFusedChunks) {
         hipLaunchKernelGGL(mass_prefix_kernel<true>, dim3(mchunks), dim3(kMassThreads), 0, st, M);
