@@ -1007,12 +1007,12 @@ __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
     // Rows of kMassThreads x kItems elements, kItems consecutive ones per
     // lane (a wave reads a contiguous run): the lane adds its elements in
     // order, one wave scan of the lanes' sums per row and quantity, then each
-    // element's prefix is the lane's running sum.  Four per lane (a quarter of
-    // the wave scans, the wave's prefixes written through LDS as contiguous
-    // runs) up to kMassFusedChunks chunks, where the scans bound the pass
-    // (tree build -10% at 131,072 particles); beyond, one per lane, a plain
-    // row scan, is faster (134 against 155 us at N = 4M).
-    constexpr int kItems = FUSED ? kMassItems : 1;
+    // element's prefix is the lane's running sum, the wave's prefixes written
+    // through LDS as contiguous runs.  Four per lane take a quarter of the
+    // wave scans of one per lane: tree build -10% at 131,072 particles, and
+    // 1.126 -> 1.095 ms at 4,194,304 since the masses are read in sorted
+    // order (while the pass gathered them, one per lane was faster there).
+    constexpr int kItems = kMassItems;
 #pragma unroll 1
     for (int r = 0; r < kMassPer / kItems; ++r) {
         const int i0 = blockIdx.x * kMassChunk + (r * kMassThreads + threadIdx.x) * kItems;
@@ -1045,10 +1045,6 @@ __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             DD p = dd_add(wpre[q][wave], wave_exclusive(incl[q], lane));
-            if constexpr (kItems == 1) {
-                if (wave0 + lane <= A.n) A.prefix[q * stride + wave0 + lane] = p;
-                continue;
-            }
 #pragma unroll
             for (int e = 0; e < kItems; ++e) {
                 stage[wave][lane * kItems + e] = p;
