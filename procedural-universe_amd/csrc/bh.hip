@@ -837,9 +837,10 @@ __device__ inline double dd_diff(DD a, DD b) {  // a - b rounded to double
 }
 
 constexpr int kMassThreads = 256;
-constexpr int kMassPer = 16;
-constexpr int kMassItems = 4;  // consecutive elements per lane in the prefix pass
-static_assert(kMassPer % kMassItems == 0, "whole rows per chunk");
+constexpr int kMassPer = 16;    // rows of kMassThreads elements in a chunk, large systems
+constexpr int kMassMinRows = 4;  // and in the smallest chunks
+constexpr int kMassItems = 4;    // consecutive elements per lane in the prefix pass
+static_assert(kMassMinRows % kMassItems == 0 && kMassPer % kMassMinRows == 0, "whole rows per chunk");
 
 constexpr int kMassChunk = kMassThreads * kMassPer;
 
@@ -852,6 +853,7 @@ struct MassArgs {
     DD* prefix;             // [4][n + 1]
     DD* chunk;              // [4][nchunks]
     int nchunks;
+    int rows;               // rows of kMassThreads elements per chunk
 };
 
 // Element k's mass and mass moments.  The chunk sums (GATHER) read the mass
@@ -900,12 +902,12 @@ __device__ inline DD wave_exclusive(DD incl, int lane) {
 
 constexpr int kMassWaves = kMassThreads / 64;
 
-// Chunk c covers elements [c * kMassChunk, (c + 1) * kMassChunk) of [0, n]
-// (element n is 0: its exclusive prefix is the total) in kMassPer rows of
-// kMassThreads consecutive elements, read with coalesced loads.  A row is
+// Chunk c covers elements [c * C, (c + 1) * C) of [0, n], C = rows *
+// kMassThreads (element n is 0: its exclusive prefix is the total), in rows
+// of kMassThreads consecutive elements, read with coalesced loads.  A row is
 // scanned by wave scans plus the totals of the waves before it.
 
-// Sum of each chunk, per quantity: each thread adds its kMassPer elements in
+// Sum of each chunk, per quantity: each thread adds its `rows` elements in
 // order (rows of kMassThreads consecutive elements, so every row's loads are
 // coalesced), then each wave's lanes are added towards lane 0 (a fixed order)
 // and thread q adds the waves' sums in order.  The chunk sums only carry the
@@ -919,8 +921,8 @@ __device__ void mass_reduce_block(const MassArgs& A, int block) {
     // every row were held in registers (144 VGPRs) and the launch's occupancy,
     // the tree's workgroups included, fell to three waves per SIMD
 #pragma unroll 4
-    for (int r = 0; r < kMassPer; ++r) {
-        const int i = block * kMassChunk + r * kMassThreads + threadIdx.x;
+    for (int r = 0; r < A.rows; ++r) {
+        const int i = (block * A.rows + r) * kMassThreads + threadIdx.x;
         double v[4];
         leaf_vals<true>(A, i, v);
 #pragma unroll
@@ -1014,8 +1016,8 @@ __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
     // order (while the pass gathered them, one per lane was faster there).
     constexpr int kItems = kMassItems;
 #pragma unroll 1
-    for (int r = 0; r < kMassPer / kItems; ++r) {
-        const int i0 = blockIdx.x * kMassChunk + (r * kMassThreads + threadIdx.x) * kItems;
+    for (int r = 0; r < A.rows / kItems; ++r) {
+        const int i0 = blockIdx.x * A.rows * kMassThreads + (r * kMassThreads + threadIdx.x) * kItems;
         double v[kItems][4];
 #pragma unroll
         for (int e = 0; e < kItems; ++e) leaf_vals<false>(A, i0 + e, v[e]);
@@ -1041,7 +1043,7 @@ __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
             carry_sh[q] = acc;
         }
         __syncthreads();
-        const int wave0 = blockIdx.x * kMassChunk + (r * kMassThreads + wave * 64) * kItems;  // the wave's run
+        const int wave0 = blockIdx.x * A.rows * kMassThreads + (r * kMassThreads + wave * 64) * kItems;  // the wave's run
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             DD p = dd_add(wpre[q][wave], wave_exclusive(incl[q], lane));
@@ -1670,6 +1672,15 @@ nbody_status bh_alloc(T*& p, size_t count) {
 
 inline int blocks_for(size_t n, int per) { return static_cast<int>((n + per - 1) / per); }
 
+// Rows per mass chunk: the fewest (small chunks give small systems more
+// workgroups in the chunk sums and the prefix pass) that keep the system
+// within kMassFusedChunks chunks, kMassPer beyond.
+inline int mass_rows(int n) {
+    for (int rows = kMassMinRows; rows < kMassPer; rows *= 2)
+        if (blocks_for(static_cast<size_t>(n) + 1, rows * kMassThreads) <= kMassFusedChunks) return rows;
+    return kMassPer;
+}
+
 float theta_below(double theta) {
     float t = static_cast<float>(theta);
     if (static_cast<double>(t) >= theta) t = std::nextafter(t, -INFINITY);
@@ -1776,7 +1787,7 @@ nbody_status bh_resize(BHState* b, size_t n, nbody_sim* s) {
     NB_TRY(bh_alloc(b->blocks, m));
     NB_TRY(bh_alloc(b->digit_totals, kRadixBins));
     NB_TRY(bh_alloc(b->prefix, 4 * (n + 1)));
-    NB_TRY(bh_alloc(b->mass_chunk, 4 * static_cast<size_t>(blocks_for(n + 1, kMassChunk))));
+    NB_TRY(bh_alloc(b->mass_chunk, 4 * static_cast<size_t>(std::max(kMassFusedChunks, blocks_for(n + 1, kMassChunk)))));
     NB_TRY(bh_alloc(b->com, m));
     NB_TRY(bh_alloc(b->thr, kThrLevels));
     b->cap = n;
@@ -1848,10 +1859,11 @@ static nbody_status bh_build(nbody_sim* s, BHState* b) {
     const int m = n - 1;
     TreeArgs T{keys, n, b->child, b->first, b->last, b->info};
     const int tree_blocks = blocks_for(m, 256);
-    const int mchunks = blocks_for(n + 1, kMassChunk);
+    const int mrows = mass_rows(n);
+    const int mchunks = blocks_for(n + 1, mrows * kMassThreads);
     // the sorted masses go to the sort's other key buffer, idle until the next step's keys
     double* mass_sorted = reinterpret_cast<double*>(b->keys[b->sorted_buf ^ 1]);
-    MassArgs M{b->pos_sorted, order, s->mass_dev, mass_sorted, n, b->prefix, b->mass_chunk, mchunks};
+    MassArgs M{b->pos_sorted, order, s->mass_dev, mass_sorted, n, b->prefix, b->mass_chunk, mchunks, mrows};
     hipLaunchKernelGGL(tree_mass_kernel, dim3(mchunks + tree_blocks), dim3(256), 0, st, T, M);
     if (mchunks <= kMassFusedChunks) {
         hipLaunchKernelGGL(mass_prefix_kernel<true>, dim3(mchunks), dim3(kMassThreads), 0, st, M);
