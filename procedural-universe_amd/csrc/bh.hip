@@ -948,7 +948,7 @@ __device__ void mass_reduce_block(const MassArgs& A, int block) {
 
 // The radix tree's nodes and the chunk sums of the mass prefix are
 // independent: one launch, the first A.nchunks workgroups reduce chunks and
-// the others build nodes (both read only the sorted particles).
+// the others build nodes (neither half reads what the other writes).
 static_assert(kMassThreads == 256, "one workgroup size for both halves");
 __global__ __launch_bounds__(256) void tree_mass_kernel(TreeArgs T, MassArgs A) {
     if (static_cast<int>(blockIdx.x) < A.nchunks) mass_reduce_block(A, blockIdx.x);
