@@ -360,15 +360,17 @@ def test_cell_export_describes_the_octree(gpu):
     assert np.all(np.diff(count)[same] < 0)
 
 
-@pytest.mark.parametrize("n", [8192, 8193, 131072, 131073, 1048575, 1048576, 1048577])
+@pytest.mark.parametrize("n", [8192, 8193, 131072, 131073, 262143, 262144, 524287, 524288,
+                               1048575, 1048576, 1048577])
 def test_sort_and_mass_paths_at_their_thresholds(gpu, oracle, n):
     """The sort's tiles hold 512 keys up to N = 131,072, 1,024 up to
     1,048,576 and 2,048 above; its scatter scans its own digit counts up to 16
-    tiles (N <= 8,192) and uses a separate scan of the counts above; the mass
-    prefix folds its chunk scan up to 256 chunks of 4096 (N + 1 <= 1,048,576).
-    On both sides of each threshold the keys and the stable order are
-    bit-exact and the cells' FP64 masses are the sums of their particles'
-    masses."""
+    tiles (N <= 8,192) and uses a separate scan of the counts above.  The mass
+    chunks hold 1,024 elements up to N = 262,143, 2,048 up to 524,287 and
+    4,096 above, and the prefix pass folds the chunk scan up to 256 chunks
+    (N + 1 <= 1,048,576).  On both sides of each threshold the keys and the
+    stable order are bit-exact and the cells' FP64 masses and centres of mass
+    are those of their particles."""
     pu = gpu
     pos, vel, mass = pu.ic_plummer(n, 0x5EED0300 + (n & 0xff))
     cfg = pu.default_config(method=pu.METHOD_BARNES_HUT, law=pu.LAW_R, softening=1e-4, theta=0.5)
@@ -388,9 +390,13 @@ def test_sort_and_mass_paths_at_their_thresholds(gpu, oracle, n):
     csum = np.concatenate([[0.0], np.cumsum(m)])
     root = cells[0]
     assert root["count"] == n and root["mass"] == pytest.approx(mass.sum(), rel=1e-12)
+    p = pos[order].astype(np.float64)
     for c in cells[:: max(1, cells.size // 500)]:
         f, k = int(c["first"]), int(c["count"])
-        assert c["mass"] == pytest.approx(m[f:f + k].sum(), rel=1e-12, abs=1e-15 * csum[-1])
+        M = m[f:f + k].sum()
+        assert c["mass"] == pytest.approx(M, rel=1e-12, abs=1e-15 * csum[-1])
+        com = (p[f:f + k] * m[f:f + k, None]).sum(0) / M
+        assert np.all(np.abs(c["com"] - com) <= 1e-6 * float(root["side"]))
 
 
 def test_sort_duplicate_heavy_above_tile_threshold(gpu, oracle):
