@@ -44,7 +44,7 @@ DIRECT_1M_VALU = {"valu_instructions_per_pair": 13.30, "rsq_per_pair": 1.004, "v
 # (profiles/bh_traverse_n4m_theta05_traffic.md) and the 512-galaxy batch step
 # (profiles/batch_step_512_galaxies_traffic.md); they hold for the default
 # workloads only.
-BH_4M_TRAFFIC_BYTES = 3.2068e+09
+BH_4M_TRAFFIC_BYTES = 3.2272e+09
 BATCH_512_TRAFFIC_BYTES = 1.6415e9
 FLOP_PER_INTERACTION = 20  # Nyland, Harris & Prins (GPU Gems 3, ch. 31) convention
 METRIC = "interactions/s & % FP32 VALU peak at N=1M; batched p50/p99 step latency"
