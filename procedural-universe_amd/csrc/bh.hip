@@ -208,10 +208,18 @@ __global__ void open_threshold_kernel(const RootParams* __restrict__ R, float th
     if (L < kThrLevels) thr[L] = open_threshold(ldexpf(R->size, -L), theta_lo);
 }
 
+// Every kernel of the tree build raises its wave priority above the
+// traversal walk's (1).  When systems are served together, a system's short,
+// dependent build launches then issue ahead of other systems' traversals on a
+// SIMD they share instead of waiting behind them (a served stream +3%); a
+// build that has the device to itself overtakes nothing.
+__device__ __forceinline__ void build_priority() { __builtin_amdgcn_s_setprio(2); }
+
 // ---- 1. root -------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void bounds_kernel(const float4* __restrict__ pos, int n,
                                                      float* __restrict__ partial) {
+    build_priority();
     __shared__ float red[6][256];
     float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
@@ -241,6 +249,7 @@ __global__ __launch_bounds__(256) void bounds_kernel(const float4* __restrict__ 
 __global__ __launch_bounds__(256) void dynamic_root_kernel(const float* __restrict__ partial, int nblocks,
                                                            RootParams* __restrict__ R, float theta_lo,
                                                            float* __restrict__ thr) {
+    build_priority();
     __shared__ float red[6][256];
     __shared__ float size_sh;
     float pm[6] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};  // min x,y,z and -max x,y,z
@@ -294,6 +303,7 @@ __global__ __launch_bounds__(256) void keys_kernel(const float4* __restrict__ po
                                                    const RootParams* __restrict__ Rp,
                                                    unsigned long long* __restrict__ keys,
                                                    unsigned* __restrict__ idx, unsigned* __restrict__ outside) {
+    build_priority();
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const RootParams R = *Rp;
@@ -331,6 +341,7 @@ template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const unsigned long long* __restrict__ keys, int n,
                                                                   int shift, unsigned* __restrict__ hist,
                                                                   int nblocks) {
+    build_priority();
     __shared__ unsigned h[kRadixBins];
     for (int d = threadIdx.x; d < kRadixBins; d += kSortThreads) h[d] = 0;
     __syncthreads();
@@ -367,6 +378,7 @@ __device__ inline unsigned block_exclusive_scan_u32(unsigned v, unsigned* sh, un
 // to totals[d]; the scatter adds the totals of the smaller digits.
 __global__ __launch_bounds__(kScanThreads2) void radix_rowscan_kernel(unsigned* __restrict__ hist, int nblocks,
                                                                       unsigned* __restrict__ totals) {
+    build_priority();
     __shared__ unsigned sh[kScanThreads2];
     const int d = blockIdx.x, t = threadIdx.x;
     unsigned* row = hist + static_cast<size_t>(d) * nblocks;
@@ -441,6 +453,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     unsigned long long* __restrict__ keys_out, unsigned* __restrict__ idx_out, int n, int shift,
     const unsigned* __restrict__ counts, const unsigned* __restrict__ totals, int nblocks,
     const float4* __restrict__ pos_in, float4* __restrict__ pos_out) {
+    build_priority();
     constexpr int kSortTile = kSortThreads * ITEMS;
     constexpr int kSortPerWave = 64 * ITEMS;  // a wavefront's share: ITEMS rounds of 64 keys
     __shared__ unsigned cnt[kSortWaves][kRadixBins];  // per-wave digit counts, then per-wave offsets in the run
@@ -773,6 +786,7 @@ __device__ __forceinline__ void write_child(const BlockArgs& A, int i, int slot,
 
 template <int LAW>
 __global__ __launch_bounds__(256) void block_kernel(BlockArgs A) {
+    build_priority();
     __shared__ int list[kBlockNodes];
     __shared__ int wave_cnt[4 * kBlockRounds];
     const int total = gather_cells(A.info, A.m, blockIdx.x * kBlockNodes, kBucketLevel, list, wave_cnt);
@@ -951,12 +965,14 @@ __device__ void mass_reduce_block(const MassArgs& A, int block) {
 // the others build nodes (neither half reads what the other writes).
 static_assert(kMassThreads == 256, "one workgroup size for both halves");
 __global__ __launch_bounds__(256) void tree_mass_kernel(TreeArgs T, MassArgs A) {
+    build_priority();
     if (static_cast<int>(blockIdx.x) < A.nchunks) mass_reduce_block(A, blockIdx.x);
     else radix_tree_block(T, blockIdx.x - A.nchunks);
 }
 
 // Exclusive scan of the chunk sums (workgroup q scans quantity q).
 __global__ __launch_bounds__(kMassThreads) void mass_chunks_kernel(MassArgs A) {
+    build_priority();
     __shared__ DD wsum[kMassWaves];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int q = blockIdx.x;
@@ -987,6 +1003,7 @@ constexpr int kMassFusedChunks = 256;
 
 template <bool FUSED>
 __global__ __launch_bounds__(kMassThreads) void mass_prefix_kernel(MassArgs A) {
+    build_priority();
     __shared__ DD wsum[4][kMassWaves];
     __shared__ DD wpre[4][kMassWaves];  // the row's sum before each wave, carry included
     __shared__ DD carry_sh[4];          // the sum before the row
@@ -1069,6 +1086,7 @@ __global__ __launch_bounds__(256) void cell_com_kernel(const DD* __restrict__ pr
                                                        const int* __restrict__ first, const int* __restrict__ last,
                                                        const int* __restrict__ info, int m, double G,
                                                        float4* __restrict__ com) {
+    build_priority();
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= m || !info_kept(info[i])) return;
     const size_t stride = static_cast<size_t>(n) + 1;
