@@ -47,6 +47,11 @@
 // with one non-empty child the mass and centre of mass do not change and the
 // side shrinks, so the reference approximates somewhere on the chain iff it
 // approximates at the chain's deepest cell, which is the cell kept here.
+//
+// Wave priorities (s_setprio, within each kernel): the build's kernels 2, the
+// traversal's walk 1 and its list evaluation 0, so that where several
+// systems share the device a short build launch or a walk's dependent loads
+// issue ahead of long runs of list arithmetic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
